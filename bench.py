@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Benchmark: adjoint gradients/s on generalized Lotka-Volterra N=10, B=65536, FP64.
+
+One step = one va_solve over the whole batch (persistent forward sweep with
+per-lane step control, terminal cost J = sum_i x_i(T), reverse sweep), inputs
+already resident in HBM.  Prints ONE JSON line (rank 0).  Also measured in the
+same run: the plain GPU form (one launch per accepted step, no lane refill,
+full (t, y, stage K) trajectory store) and the CPU oracle on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "adjoint gradients/sec, GLV N=10 B=65536 FP64 (vs plain GPU form and CPU AADC)"
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix peaks are equal)
+HBM_PEAK_GBS = 8000.0
+# accepted-step capacity per instance: covers 99.99% of the healthy instances of this
+# workload; instances that need more (and the ~0.1% that blow up in finite time)
+# end with status MAX_STEPS in the GPU path and in the oracle alike and are not
+# counted as gradients.
+MAX_STEPS = 1024
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_solver(solver, y0, p, cost, steps, warmup, torch):
+    out = None
+    for _ in range(warmup):
+        out = solver.solve(y0, p, cost, out=out)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        out = solver.solve(y0, p, cost, out=out)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / steps, out
+
+
+def kernel_times(solver, y0, p, reps, torch):
+    """Average device time of the forward launch and of the reverse launch, with HIP
+    events recorded on the stream the library launches on (torch's current stream)."""
+    N, B = y0.shape
+    y = y0.clone()
+    lam = torch.ones((1, N, B), dtype=torch.float64, device=y0.device)
+    mu = torch.zeros((1, p.shape[0], B), dtype=torch.float64, device=y0.device)
+    n_acc = torch.empty(B, dtype=torch.int32, device=y0.device)
+    n_rej = torch.empty_like(n_acc)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tf, tr = [], []
+    for _ in range(reps):
+        y.copy_(y0)
+        lam.fill_(1.0)
+        ev[0].record()
+        solver.forward(y, p, n_accepted=n_acc, n_rejected=n_rej)
+        ev[1].record()
+        solver.adjoint(p, lam, mu)
+        ev[2].record()
+        torch.cuda.synchronize()
+        tf.append(ev[0].elapsed_time(ev[1]))
+        tr.append(ev[1].elapsed_time(ev[2]))
+    return float(np.mean(tf)), float(np.mean(tr)), n_acc.cpu().numpy(), n_rej.cpu().numpy()
+
+
+def cpu_baseline(w, inputs, target_s=10.0):
+    """The CPU oracle timed on the host cores on a bounded prefix of the same batch."""
+    import oracle as O
+    nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    B = inputs["y0"].shape[1]
+    pb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
+                   abs_tol=w.tol, rel_tol=w.tol, max_steps=4096)
+    probe = min(B, 512 * nthreads)
+    t = time.perf_counter()
+    O.solve_batch(pb, inputs["y0"][:, :probe].copy(), inputs["p"][:, :probe].copy(), w.cost,
+                  nthreads=nthreads)
+    dt = time.perf_counter() - t
+    k = int(min(B, max(probe, probe * target_s / max(dt, 1e-6))))
+    t = time.perf_counter()
+    O.solve_batch(pb, inputs["y0"][:, :k].copy(), inputs["p"][:, :k].copy(), w.cost,
+                  nthreads=nthreads)
+    dt = time.perf_counter() - t
+    return {"value": k / dt, "unit": "gradients/s", "cores": nthreads, "kind": "port",
+            "sample": f"first {k} of {B} instances (same seeded inputs), {dt:.1f} s wall, "
+                      f"{nthreads} threads"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(instances would shard across GPUs with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import flops as FL
+    from vectorizedadjoint_amd import inputs as I
+
+    w = I.WORKLOADS["glv10"]
+    B = w.batch
+    log(f"[bench] generating {w.name} B={B}")
+    inp = I.make_inputs(w, B)
+    y0 = torch.from_numpy(inp["y0"]).cuda()
+    p = torch.from_numpy(inp["p"]).cuda()
+
+    pb = va.Problem.from_workload(w, max_steps=MAX_STEPS)
+    log("[bench] persistent form")
+    with va.Solver(pb) as s:
+        sec, out = time_solver(s, y0, p, w.cost, a.steps, a.warmup, torch)
+        st = out["status"].cpu().numpy()
+        tf, tr, n_acc, n_rej = kernel_times(s, y0, p, 5, torch)
+    n_ok = int((st == 0).sum())
+    fl_f = FL.forward_flops(w.system, w.n, w.tableau, n_acc, n_rej)
+    fl_r = FL.reverse_flops(w.system, w.n, w.tableau, n_acc)
+    dom = ("reverse", tr, fl_r) if tr >= tf else ("forward", tf, fl_f)
+    achieved = dom[2] / (dom[1] * 1e-3) / 1e12
+
+    plain = None
+    if not a.no_plain:
+        log("[bench] plain GPU form")
+        with va.Solver(va.Problem.from_workload(w, max_steps=MAX_STEPS, mode=va.MODE_PLAIN)) as s2:
+            psec, _ = time_solver(s2, y0, p, w.cost, max(1, a.steps // 4), 1, torch)
+        plain = {"value": n_ok / psec, "ms_per_step": psec * 1e3,
+                 "form": "one launch per accepted step, no refill, full (t, y, K) store"}
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline")
+        cpu = cpu_baseline(w, inp)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_glv10.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom[0])
+
+    res = {
+        "metric": METRIC, "value": n_ok / sec, "unit": "gradients/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": sec * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 inputs, random-init parameters)",
+        "config": {"workload": "glv10", "system": "generalized Lotka-Volterra", "n_state": w.n,
+                   "n_param": w.np, "batch": B, "tableau": "cash_karp54", "tol": w.tol,
+                   "t_span": [w.t0, w.t1], "cost": "sum_i x_i(T)",
+                   "instances_ok": n_ok, "max_steps": MAX_STEPS},
+        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (MI355X FP64 vector = "
+                     "matrix peak)", "kernel": dom[0], "achieved": achieved,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel_ms": dom[1], "forward_ms": tf, "reverse_ms": tr,
+                     "flop_per_launch": dom[2]},
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "speedup_vs_plain": (plain["ms_per_step"] / (sec * 1e3)) if plain else None,
+        "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
+                               "accepted_max": int(n_acc.max()),
+                               "rejected_mean": float(n_rej.mean())},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

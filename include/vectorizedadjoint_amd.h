@@ -1,0 +1,160 @@
+/*
+ * vectorizedadjoint_amd.h -- C ABI of the MI355X (gfx950) batched explicit
+ * Runge-Kutta forward sweep and discrete-adjoint reverse sweep, FP64.
+ *
+ * The reference (RuiMartins1996/VectorizedAdjoint) is a header-only C++ library
+ * whose entry points are templates in namespace vectorizedadjoint.  This ABI
+ * is what those entry points bind to here; the C++ drop-in headers under
+ * include/vectorizedadjoint/ keep the reference's names and call through it.
+ * Each entry point below cites the reference interface it replaces.
+ *
+ * Conventions
+ *   - One handle = one problem (system, tableau, tolerances, time span) over a
+ *     batch of B independent instances (parameter / initial-condition sets).
+ *     A handle owns its device trajectory store; it is not thread-safe.
+ *   - Every array argument is a DEVICE pointer (hipMalloc'd memory of the
+ *     current device), laid out structure-of-arrays with the instance index
+ *     fastest: y[i][b] at y[i*B + b], p[k][b] at p[k*B + b],
+ *     lambda[o][i][b] at lambda[(o*N + i)*B + b].
+ *   - `stream` is a hipStream_t (NULL = the null stream).  Calls are
+ *     stream-ordered and return without synchronising, except where noted.
+ *   - Functions return va_status and never throw; va_last_error() gives a
+ *     message for the calling thread.
+ */
+#ifndef VECTORIZEDADJOINT_AMD_H
+#define VECTORIZEDADJOINT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VA_ABI_VERSION 1
+
+typedef enum va_status {
+    VA_OK = 0,
+    VA_ERR_INVALID = 1,        /* bad argument / problem description            */
+    VA_ERR_UNSUPPORTED = 2,    /* no kernels for this (system, N, tableau)     */
+    VA_ERR_HIP = 3,            /* HIP runtime error                             */
+    VA_ERR_OUT_OF_MEMORY = 4,
+    VA_ERR_STATE = 5,          /* call order violated (e.g. adjoint before forward) */
+    VA_ERR_LANE_FAILURE = 6    /* reserved: some instance failed (see lane status) */
+} va_status;
+
+/* Built-in systems (device right-hand sides with hand-written VJPs). */
+typedef enum va_system {
+    VA_SYSTEM_VAN_DER_POL = 1,         /* examples/VanDerPol/main.cpp:39-43; p = (mu)          */
+    VA_SYSTEM_OSCILLATOR = 2,          /* damped oscillator, p = (k, c)                         */
+    VA_SYSTEM_OSCILLATOR_FIXED_K = 3,  /* examples/HarmonicOscillator/main.cpp:25-29; p = (mu),
+                                          k = va_problem.system_constant                      */
+    VA_SYSTEM_LOTKA_VOLTERRA = 4       /* examples/GeneralizedLotkaVolterra/main.cpp:105-119;
+                                          p = (r[N], A[N][N] row-major), P = N + N*N          */
+} va_system;
+
+/* Tableaus: ButcherTable.hpp:39-251 (Euler, RK4, Cash-Karp 5(4), Fehlberg 7(8))
+ * plus Dormand-Prince 5(4), which the reference rejects (ButcherTable.hpp:247-250). */
+typedef enum va_tableau {
+    VA_TABLEAU_EULER = 1,
+    VA_TABLEAU_RK4 = 2,
+    VA_TABLEAU_CASH_KARP54 = 3,
+    VA_TABLEAU_DOPRI5 = 4,
+    VA_TABLEAU_FEHLBERG78 = 5
+} va_tableau;
+
+/* Terminal cost functionals evaluated on the device by va_solve. */
+typedef enum va_cost {
+    VA_COST_SUM = 1,           /* J = sum_i y_i(T)                                   */
+    VA_COST_SQUARED_NORM = 2,  /* J = sum_i y_i(T)^2                                 */
+    VA_COST_LINEAR = 3         /* J = sum_i w_i y_i(T), w = cost_weights (device, N) */
+} va_cost;
+
+typedef enum va_mode {
+    VA_MODE_PERSISTENT = 0,    /* persistent kernels with lane refill (default)     */
+    VA_MODE_PLAIN = 1          /* one launch per accepted step, no refill, full
+                                  (t, y, stage K) store: the baseline GPU form       */
+} va_mode;
+
+/* Per-instance status written by va_forward / va_solve. */
+typedef enum va_lane_status {
+    VA_LANE_OK = 0,
+    VA_LANE_MAX_FAIL = 1,      /* more than max_attempts tries in one step
+                                  (odeint failed_step_checker -> step_adjustment_error) */
+    VA_LANE_NONFINITE = 2,     /* y(T) not finite                                    */
+    VA_LANE_MAX_STEPS = 3      /* more accepted steps than max_steps                 */
+} va_lane_status;
+
+typedef struct va_problem {
+    int32_t abi_version;       /* VA_ABI_VERSION                                      */
+    int32_t system;            /* va_system                                           */
+    int32_t tableau;           /* va_tableau                                          */
+    int32_t adaptive;          /* 1: controlled loop (runge_kutta.hpp:74-118),
+                                  0: fixed step (runge_kutta.hpp:36-72)               */
+    int32_t n_state;           /* N (Nin)                                             */
+    int32_t n_param;           /* P (Npar), must match the system                     */
+    int64_t batch;             /* B instances                                         */
+    double t0, t1, dt0;        /* start/end time, initial (or fixed) step             */
+    double abs_tol, rel_tol;   /* make_controlled(abs, rel)                           */
+    double system_constant;    /* non-differentiated constant (VA_SYSTEM_OSCILLATOR_FIXED_K) */
+    int32_t max_attempts;      /* tries per step before VA_LANE_MAX_FAIL (odeint: 500) */
+    int32_t max_steps;         /* accepted-step capacity of each stored trajectory    */
+    int32_t mode;              /* va_mode                                             */
+    int32_t block_size;        /* threads per block, 0 = tuned                        */
+    int64_t lanes;             /* persistent lanes in flight, 0 = tuned               */
+} va_problem;
+
+typedef struct va_handle va_handle;
+
+/* Fills defaults: odeint limits (500 tries), dt0 = 1e-3, tolerances 1e-6. */
+void va_problem_init(va_problem *pb);
+
+/* Driver(Nin, Nout, Npar) + constructDriverButcherTableau + recordDriverRHSFunction
+ * (lib/include/Driver.hpp:39-42,87-100): validates the problem, selects the
+ * device kernels for (system, N, tableau) and allocates the trajectory store
+ * (B * (max_steps+1) rows).  */
+va_status va_create(const va_problem *pb, va_handle **out);
+
+/* delete_driver_handle (lib/include/Driver.hpp:81-85). */
+void va_destroy(va_handle *h);
+
+/* runge_kutta(stepper, system, x0, p, t0, t1, dt, driver)
+ * (lib/include/runge_kutta.hpp:47-59 -> detail/runge_kutta.hpp:36-118):
+ * y: in y0 [N][B], out y(T) (in place, as x0 in the reference); p: [P][B].
+ * Stores (t_n, y_n) before every accepted step and at the end.  n_accepted,
+ * n_rejected, lane_status: [B] int32, each may be NULL. */
+va_status va_forward(va_handle *h, double *y, const double *p, int32_t *n_accepted,
+                     int32_t *n_rejected, int32_t *lane_status, void *stream);
+
+/* setCostGradients + adjointSolve (lib/include/Driver.hpp:103-114,
+ * lib/include/backpropagation.hpp:18-43 -> detail/backpropagation.hpp:280-348)
+ * over the trajectory stored by the last va_forward on this handle:
+ * lambda [nout][N][B]: in dJ/dy(T), out dJ/dy0 (overwritten);
+ * mu [nout][P][B]: dJ/dp accumulated (+=).  Instances whose forward failed get NaN. */
+va_status va_adjoint(va_handle *h, const double *p, int32_t nout, double *lambda, double *mu,
+                     void *stream);
+
+/* computeSensitivityMatrix (lib/include/backpropagation.hpp:67-87,
+ * detail/backpropagation.hpp:373-434): S [N][P][B], S[i][k][b] = d y_i(T) / d p_k.
+ * Allocates a temporary seed buffer of N*N*B doubles. */
+va_status va_sensitivity_matrix(va_handle *h, const double *p, double *S, void *stream);
+
+/* Forward + terminal cost + adjoint in one call (no reference counterpart: the
+ * reference's drivers chain runge_kutta / setCostGradients / adjointSolve with
+ * the cost written by hand, examples/VanDerPol/main.cpp:98-117).  Outputs: J [B], yT [N][B],
+ * dJdy0 [N][B], dJdp [P][B] (overwritten), counts and status [B] (may be NULL).
+ * cost_weights: [N] device array for VA_COST_LINEAR, else NULL. */
+va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost,
+                   const double *cost_weights, double *J, double *yT, double *dJdy0,
+                   double *dJdp, int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
+                   void *stream);
+
+/* Largest accepted-step count of the last forward pass (synchronises the stream). */
+va_status va_max_steps_seen(va_handle *h, int32_t *out, void *stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *va_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

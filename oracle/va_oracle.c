@@ -1,0 +1,808 @@
+/*
+ * va_oracle.c -- CPU restatement of VectorizedAdjoint's hot path (test
+ * infrastructure; see va_oracle.h for the scope and the file:line map).
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off -mfma).
+ */
+#define _GNU_SOURCE
+#include "va_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Controller pow.  Own implementation, written from the textbook methods:   */
+/*   log(x) = e ln2 + 2 atanh(s),  s = (m-1)/(m+1),  m in [sqrt(1/2), sqrt2) */
+/*   exp(t) = 2^k exp(r),  r = t - k ln2 (Cody-Waite two-constant split),    */
+/*            exp(r) by its Taylor series to degree 13 (|r| <= ln2/2).       */
+/* Only IEEE basic operations, fma, frexp, ldexp and rint are used, so the   */
+/* HIP kernels (which use the same sequence) obtain the same bits.           */
+/* ------------------------------------------------------------------------ */
+#define ORC_LN2_HI 0x1.62e42fee00000p-1 /* ln2 truncated to 32 significant bits */
+#define ORC_LN2_LO 0x1.a39ef35793c76p-33
+#define ORC_INV_LN2 0x1.71547652b82fep+0
+#define ORC_SQRT_HALF 0x1.6a09e667f3bcdp-1
+
+static double orc_log_pos(double x)
+{
+    int e;
+    double m = frexp(x, &e); /* x = m 2^e, m in [0.5, 1) */
+    if (m < ORC_SQRT_HALF) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double z = s * s;
+    /* 1 + z/3 + z^2/5 + ... + z^11/23 */
+    double q = 1.0 / 23.0;
+    q = fma(q, z, 1.0 / 21.0);
+    q = fma(q, z, 1.0 / 19.0);
+    q = fma(q, z, 1.0 / 17.0);
+    q = fma(q, z, 1.0 / 15.0);
+    q = fma(q, z, 1.0 / 13.0);
+    q = fma(q, z, 1.0 / 11.0);
+    q = fma(q, z, 1.0 / 9.0);
+    q = fma(q, z, 1.0 / 7.0);
+    q = fma(q, z, 1.0 / 5.0);
+    q = fma(q, z, 1.0 / 3.0);
+    double zq = z * q;
+    double s2 = s + s;
+    double lm = fma(s2, zq, s2); /* 2s (1 + z q) */
+    double de = (double)e;
+    return fma(de, ORC_LN2_HI, fma(de, ORC_LN2_LO, lm));
+}
+
+static double orc_exp(double t)
+{
+    if (t != t) return t;
+    if (t > 709.8) return INFINITY;
+    if (t < -745.2) return 0.0;
+    double k = rint(t * ORC_INV_LN2);
+    double r = fma(-k, ORC_LN2_HI, t);
+    r = fma(-k, ORC_LN2_LO, r);
+    double q = 1.0 / 6227020800.0; /* 1/13! */
+    q = fma(q, r, 1.0 / 479001600.0);
+    q = fma(q, r, 1.0 / 39916800.0);
+    q = fma(q, r, 1.0 / 3628800.0);
+    q = fma(q, r, 1.0 / 362880.0);
+    q = fma(q, r, 1.0 / 40320.0);
+    q = fma(q, r, 1.0 / 5040.0);
+    q = fma(q, r, 1.0 / 720.0);
+    q = fma(q, r, 1.0 / 120.0);
+    q = fma(q, r, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    q = fma(q, r, 1.0);
+    q = fma(q, r, 1.0);
+    return ldexp(q, (int)k);
+}
+
+double orc_pow(double x, double y)
+{
+    if (x != x || y != y) return x + y;
+    if (y == 0.0) return 1.0;
+    if (x == 0.0) return y < 0.0 ? INFINITY : 0.0;
+    if (x < 0.0) return NAN;
+    if (isinf(x)) return y < 0.0 ? 0.0 : INFINITY;
+    return orc_exp(y * orc_log_pos(x));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Butcher tableaus (ButcherTable.hpp:39-251 plus DOPRI5, which the         */
+/* reference leaves unsupported at ButcherTable.hpp:247-250).  Coefficients  */
+/* are the published ones: classic RK4; Cash & Karp (1990); Dormand & Prince */
+/* (1980); Fehlberg (1968, NASA TR R-287).  db = b - b_embedded is rounded   */
+/* the way odeint 1.74 rounds it (difference of two rounded quotients).      */
+/* ------------------------------------------------------------------------ */
+static void tab_clear(orc_tableau *t)
+{
+    memset(t, 0, sizeof(*t));
+}
+
+int orc_get_tableau(int id, orc_tableau *t)
+{
+    tab_clear(t);
+    switch (id) {
+    case ORC_TAB_EULER:
+        t->s = t->s_fwd = 1;
+        t->order = 1;
+        t->b[0] = 1.0;
+        return 0;
+    case ORC_TAB_RK4:
+        t->s = t->s_fwd = 4;
+        t->order = 4;
+        t->a[1][0] = 1.0 / 2.0;
+        t->a[2][1] = 1.0 / 2.0;
+        t->a[3][2] = 1.0;
+        t->b[0] = 1.0 / 6.0; t->b[1] = 1.0 / 3.0; t->b[2] = 1.0 / 3.0; t->b[3] = 1.0 / 6.0;
+        t->c[1] = 1.0 / 2.0; t->c[2] = 1.0 / 2.0; t->c[3] = 1.0;
+        return 0;
+    case ORC_TAB_CASH_KARP54:
+        t->s = t->s_fwd = 6;
+        t->order = 5;
+        t->err_order = 4;
+        t->has_err = 1;
+        t->a[1][0] = 1.0 / 5.0;
+        t->a[2][0] = 3.0 / 40.0; t->a[2][1] = 9.0 / 40.0;
+        t->a[3][0] = 3.0 / 10.0; t->a[3][1] = -9.0 / 10.0; t->a[3][2] = 6.0 / 5.0;
+        t->a[4][0] = -11.0 / 54.0; t->a[4][1] = 5.0 / 2.0; t->a[4][2] = -70.0 / 27.0;
+        t->a[4][3] = 35.0 / 27.0;
+        t->a[5][0] = 1631.0 / 55296.0; t->a[5][1] = 175.0 / 512.0; t->a[5][2] = 575.0 / 13824.0;
+        t->a[5][3] = 44275.0 / 110592.0; t->a[5][4] = 253.0 / 4096.0;
+        t->b[0] = 37.0 / 378.0; t->b[2] = 250.0 / 621.0; t->b[3] = 125.0 / 594.0;
+        t->b[5] = 512.0 / 1771.0;
+        t->db[0] = 37.0 / 378.0 - 2825.0 / 27648.0;
+        t->db[2] = 250.0 / 621.0 - 18575.0 / 48384.0;
+        t->db[3] = 125.0 / 594.0 - 13525.0 / 55296.0;
+        t->db[4] = -277.0 / 14336.0;
+        t->db[5] = 512.0 / 1771.0 - 1.0 / 4.0;
+        t->c[1] = 1.0 / 5.0; t->c[2] = 3.0 / 10.0; t->c[3] = 3.0 / 5.0; t->c[4] = 1.0;
+        t->c[5] = 7.0 / 8.0;
+        return 0;
+    case ORC_TAB_DOPRI5:
+        t->s = 6;
+        t->s_fwd = 7;
+        t->order = 5;
+        t->err_order = 4;
+        t->has_err = 1;
+        t->fsal = 1;
+        t->a[1][0] = 1.0 / 5.0;
+        t->a[2][0] = 3.0 / 40.0; t->a[2][1] = 9.0 / 40.0;
+        t->a[3][0] = 44.0 / 45.0; t->a[3][1] = -56.0 / 15.0; t->a[3][2] = 32.0 / 9.0;
+        t->a[4][0] = 19372.0 / 6561.0; t->a[4][1] = -25360.0 / 2187.0;
+        t->a[4][2] = 64448.0 / 6561.0; t->a[4][3] = -212.0 / 729.0;
+        t->a[5][0] = 9017.0 / 3168.0; t->a[5][1] = -355.0 / 33.0; t->a[5][2] = 46732.0 / 5247.0;
+        t->a[5][3] = 49.0 / 176.0; t->a[5][4] = -5103.0 / 18656.0;
+        t->a[6][0] = 35.0 / 384.0; t->a[6][2] = 500.0 / 1113.0; t->a[6][3] = 125.0 / 192.0;
+        t->a[6][4] = -2187.0 / 6784.0; t->a[6][5] = 11.0 / 84.0;
+        t->b[0] = 35.0 / 384.0; t->b[2] = 500.0 / 1113.0; t->b[3] = 125.0 / 192.0;
+        t->b[4] = -2187.0 / 6784.0; t->b[5] = 11.0 / 84.0;
+        t->db[0] = 35.0 / 384.0 - 5179.0 / 57600.0;
+        t->db[2] = 500.0 / 1113.0 - 7571.0 / 16695.0;
+        t->db[3] = 125.0 / 192.0 - 393.0 / 640.0;
+        t->db[4] = -2187.0 / 6784.0 - -92097.0 / 339200.0;
+        t->db[5] = 11.0 / 84.0 - 187.0 / 2100.0;
+        t->db[6] = -1.0 / 40.0;
+        t->c[1] = 1.0 / 5.0; t->c[2] = 3.0 / 10.0; t->c[3] = 4.0 / 5.0; t->c[4] = 8.0 / 9.0;
+        t->c[5] = 1.0; t->c[6] = 1.0;
+        return 0;
+    case ORC_TAB_FEHLBERG78:
+        t->s = t->s_fwd = 13;
+        t->order = 8;
+        t->err_order = 7;
+        t->has_err = 1;
+        t->a[1][0] = 2.0 / 27.0;
+        t->a[2][0] = 1.0 / 36.0; t->a[2][1] = 1.0 / 12.0;
+        t->a[3][0] = 1.0 / 24.0; t->a[3][2] = 1.0 / 8.0;
+        t->a[4][0] = 5.0 / 12.0; t->a[4][2] = -25.0 / 16.0; t->a[4][3] = 25.0 / 16.0;
+        t->a[5][0] = 1.0 / 20.0; t->a[5][3] = 1.0 / 4.0; t->a[5][4] = 1.0 / 5.0;
+        t->a[6][0] = -25.0 / 108.0; t->a[6][3] = 125.0 / 108.0; t->a[6][4] = -65.0 / 27.0;
+        t->a[6][5] = 125.0 / 54.0;
+        t->a[7][0] = 31.0 / 300.0; t->a[7][4] = 61.0 / 225.0; t->a[7][5] = -2.0 / 9.0;
+        t->a[7][6] = 13.0 / 900.0;
+        t->a[8][0] = 2.0; t->a[8][3] = -53.0 / 6.0; t->a[8][4] = 704.0 / 45.0;
+        t->a[8][5] = -107.0 / 9.0; t->a[8][6] = 67.0 / 90.0; t->a[8][7] = 3.0;
+        t->a[9][0] = -91.0 / 108.0; t->a[9][3] = 23.0 / 108.0; t->a[9][4] = -976.0 / 135.0;
+        t->a[9][5] = 311.0 / 54.0; t->a[9][6] = -19.0 / 60.0; t->a[9][7] = 17.0 / 6.0;
+        t->a[9][8] = -1.0 / 12.0;
+        t->a[10][0] = 2383.0 / 4100.0; t->a[10][3] = -341.0 / 164.0; t->a[10][4] = 4496.0 / 1025.0;
+        t->a[10][5] = -301.0 / 82.0; t->a[10][6] = 2133.0 / 4100.0; t->a[10][7] = 45.0 / 82.0;
+        t->a[10][8] = 45.0 / 164.0; t->a[10][9] = 18.0 / 41.0;
+        t->a[11][0] = 3.0 / 205.0; t->a[11][5] = -6.0 / 41.0; t->a[11][6] = -3.0 / 205.0;
+        t->a[11][7] = -3.0 / 41.0; t->a[11][8] = 3.0 / 41.0; t->a[11][9] = 6.0 / 41.0;
+        t->a[12][0] = -1777.0 / 4100.0; t->a[12][3] = -341.0 / 164.0; t->a[12][4] = 4496.0 / 1025.0;
+        t->a[12][5] = -289.0 / 82.0; t->a[12][6] = 2193.0 / 4100.0; t->a[12][7] = 51.0 / 82.0;
+        t->a[12][8] = 33.0 / 164.0; t->a[12][9] = 12.0 / 41.0; t->a[12][11] = 1.0;
+        t->b[5] = 34.0 / 105.0; t->b[6] = 9.0 / 35.0; t->b[7] = 9.0 / 35.0; t->b[8] = 9.0 / 280.0;
+        t->b[9] = 9.0 / 280.0; t->b[11] = 41.0 / 840.0; t->b[12] = 41.0 / 840.0;
+        t->db[0] = 0.0 - 41.0 / 840.0;
+        t->db[10] = 0.0 - 41.0 / 840.0;
+        t->db[11] = 41.0 / 840.0;
+        t->db[12] = 41.0 / 840.0;
+        t->c[1] = 2.0 / 27.0; t->c[2] = 1.0 / 9.0; t->c[3] = 1.0 / 6.0; t->c[4] = 5.0 / 12.0;
+        t->c[5] = 1.0 / 2.0; t->c[6] = 5.0 / 6.0; t->c[7] = 1.0 / 6.0; t->c[8] = 2.0 / 3.0;
+        t->c[9] = 1.0 / 3.0; t->c[10] = 1.0; t->c[11] = 0.0; t->c[12] = 1.0;
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Systems: right-hand sides as the reference examples define them, and     */
+/* their hand-derived vector-Jacobian products (these replace the AADC tape, */
+/* AadData.hpp:124-171,291-330).                                             */
+/* ------------------------------------------------------------------------ */
+void orc_rhs(int system, int n, const double *x, const double *p, double t, double K,
+             double *f)
+{
+    (void)t;
+    switch (system) {
+    case ORC_SYS_VAN_DER_POL: { /* examples/VanDerPol/main.cpp:41-42 */
+        double x0 = x[0], x1 = x[1];
+        f[0] = x1;
+        f[1] = p[0] * ((1.0 - x0 * x0) * x1 - x0);
+        break;
+    }
+    case ORC_SYS_OSCILLATOR: /* examples/HarmonicOscillator/main.cpp:27-28 with k a parameter */
+        f[0] = x[1];
+        f[1] = -p[0] * x[0] - p[1] * x[1];
+        break;
+    case ORC_SYS_OSCILLATOR_FIXED_K: /* examples/HarmonicOscillator/main.cpp:27-28 */
+        f[0] = x[1];
+        f[1] = -K * x[0] - p[0] * x[1];
+        break;
+    case ORC_SYS_LOTKA_VOLTERRA:
+    case ORC_SYS_LOTKA_VOLTERRA_SHARED: /* examples/GeneralizedLotkaVolterra/main.cpp:109-118 */
+        for (int i = 0; i < n; ++i) {
+            const double *Ai = p + (size_t)n * (i + 1);
+            double g = p[i];
+            for (int j = 0; j < n; ++j) g = fma(Ai[j], x[j], g);
+            f[i] = x[i] * g;
+        }
+        break;
+    default:
+        for (int i = 0; i < n; ++i) f[i] = NAN;
+    }
+}
+
+void orc_vjp(int system, int n, int np, const double *x, const double *p, double t, double K,
+             const double *w, double *wJx, double *wJp)
+{
+    (void)t;
+    (void)np;
+    switch (system) {
+    case ORC_SYS_VAN_DER_POL: {
+        double x0 = x[0], x1 = x[1];
+        double wm = w[1] * p[0];
+        double q = 1.0 - x0 * x0;
+        wJx[0] = wm * fma(-2.0 * x0, x1, -1.0);
+        wJx[1] = fma(wm, q, w[0]);
+        wJp[0] = w[1] * fma(q, x1, -x0);
+        break;
+    }
+    case ORC_SYS_OSCILLATOR:
+        wJx[0] = -(w[1] * p[0]);
+        wJx[1] = w[0] - w[1] * p[1];
+        wJp[0] = -(w[1] * x[0]);
+        wJp[1] = -(w[1] * x[1]);
+        break;
+    case ORC_SYS_OSCILLATOR_FIXED_K:
+        wJx[0] = -(w[1] * K);
+        wJx[1] = w[0] - w[1] * p[0];
+        wJp[0] = -(w[1] * x[1]);
+        break;
+    case ORC_SYS_LOTKA_VOLTERRA:
+    case ORC_SYS_LOTKA_VOLTERRA_SHARED: {
+        /* f_i = x_i g_i, g_i = r_i + sum_j A_ij x_j
+         * wJx_j = w_j g_j + sum_i (w_i x_i) A_ij;  r_bar_i = w_i x_i;  A_bar_ij = w_i x_i x_j */
+        double u[ORC_MAX_N];
+        for (int i = 0; i < n; ++i) u[i] = w[i] * x[i];
+        for (int j = 0; j < n; ++j) {
+            const double *Aj = p + (size_t)n * (j + 1);
+            double g = p[j];
+            for (int k = 0; k < n; ++k) g = fma(Aj[k], x[k], g);
+            double acc = w[j] * g;
+            for (int i = 0; i < n; ++i) acc = fma(u[i], p[(size_t)n * (i + 1) + j], acc);
+            wJx[j] = acc;
+        }
+        for (int i = 0; i < n; ++i) {
+            wJp[i] = u[i];
+            double *Ab = wJp + (size_t)n * (i + 1);
+            for (int j = 0; j < n; ++j) Ab[j] = u[i] * x[j];
+        }
+        break;
+    }
+    default:
+        for (int i = 0; i < n; ++i) wJx[i] = NAN;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* odeint helpers (Boost 1.74 util/detail/less_with_sign.hpp, restated)     */
+/* ------------------------------------------------------------------------ */
+static int less_with_sign(double t1, double t2, double dt)
+{
+    if (dt > 0.0) return t2 - t1 > DBL_EPSILON;
+    return t1 - t2 > DBL_EPSILON;
+}
+
+static int less_eq_with_sign(double t1, double t2, double dt)
+{
+    if (dt > 0.0) return t1 - t2 <= DBL_EPSILON;
+    return t2 - t1 <= DBL_EPSILON;
+}
+
+/* Stage sums, identical operation sequence to the HIP kernels:
+ *   X_m  = y + sum_{j<m, a_mj != 0} (a_mj h) K_j   (fma chain in j order)
+ *   y'   = y + sum_{b_j != 0} (b_j h) K_j
+ *   xerr = 0 + sum_{db_j != 0} (db_j h) K_j                                   */
+static void stage_input(const orc_tableau *tb, int m, int n, double h, const double *y,
+                        double K[][ORC_MAX_N], double *X)
+{
+    for (int i = 0; i < n; ++i) X[i] = y[i];
+    for (int j = 0; j < m; ++j) {
+        double a = tb->a[m][j];
+        if (a == 0.0) continue;
+        double ah = a * h;
+        for (int i = 0; i < n; ++i) X[i] = fma(ah, K[j][i], X[i]);
+    }
+}
+
+typedef struct {
+    double K[ORC_MAX_STAGES][ORC_MAX_N];
+    double X[ORC_MAX_N];
+    double ynew[ORC_MAX_N];
+    double xerr[ORC_MAX_N];
+} orc_work;
+
+/* One attempt from (y, K0 = f(y)) with step h: fills w->K[1..s_fwd-1],
+ * w->ynew and, when the tableau has an embedded pair, w->xerr. */
+static int attempt(const orc_problem *pb, const orc_tableau *tb, const double *y,
+                   const double *p, double t, double h, orc_work *w)
+{
+    int n = pb->n;
+    int evals = 0;
+    for (int m = 1; m < tb->s_fwd; ++m) {
+        stage_input(tb, m, n, h, y, w->K, w->X);
+        double tm = t + tb->c[m] * h;
+        orc_rhs(pb->system, n, w->X, p, tm, pb->sys_const, w->K[m]);
+        ++evals;
+    }
+    for (int i = 0; i < n; ++i) w->ynew[i] = y[i];
+    for (int j = 0; j < tb->s; ++j) {
+        double bb = tb->b[j];
+        if (bb == 0.0) continue;
+        double bh = bb * h;
+        for (int i = 0; i < n; ++i) w->ynew[i] = fma(bh, w->K[j][i], w->ynew[i]);
+    }
+    if (tb->has_err) {
+        for (int i = 0; i < n; ++i) w->xerr[i] = 0.0;
+        for (int j = 0; j < tb->s_fwd; ++j) {
+            double d = tb->db[j];
+            if (d == 0.0) continue;
+            double dh = d * h;
+            for (int i = 0; i < n; ++i) w->xerr[i] = fma(dh, w->K[j][i], w->xerr[i]);
+        }
+    }
+    return evals;
+}
+
+void orc_rk_step(const orc_problem *pb, const double *y, const double *p, double t, double h,
+                 double *out, double *xerr)
+{
+    orc_tableau tb;
+    orc_work w;
+    orc_get_tableau(pb->tableau, &tb);
+    orc_rhs(pb->system, pb->n, y, p, t, pb->sys_const, w.K[0]);
+    attempt(pb, &tb, y, p, t, h, &w);
+    for (int i = 0; i < pb->n; ++i) out[i] = w.ynew[i];
+    if (xerr)
+        for (int i = 0; i < pb->n; ++i) xerr[i] = tb.has_err ? w.xerr[i] : 0.0;
+}
+
+/* default_error_checker::error (a_x = a_dxdt = 1), norm_inf with std::max
+ * semantics (a NaN component never replaces the running maximum). */
+static double error_norm(int n, double atol, double rtol, double h, const double *y,
+                         const double *dydt, const double *xerr)
+{
+    double m = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double den = fma(rtol, fma(h, fabs(dydt[i]), fabs(y[i])), atol);
+        double e = fabs(xerr[i]) / den;
+        m = (m < e) ? e : m;
+    }
+    return m;
+}
+
+/* default_step_adjuster (max_dt = 0) */
+static double decrease_step(double h, double err, int err_order)
+{
+    double f = 0.9 * orc_pow(err, -1.0 / (double)(err_order - 1));
+    f = (f < 0.2) ? 0.2 : f;
+    return h * f;
+}
+
+static double min_err_for_order(int q)
+{
+    double d = 1.0;
+    for (int i = 0; i < q; ++i) d = d * 5.0;
+    return 1.0 / d; /* pow(5, -q), exact quotient correctly rounded */
+}
+
+static double increase_step(double h, double err, int order)
+{
+    if (err < 0.5) {
+        double lo = min_err_for_order(order);
+        err = (lo < err) ? err : lo;
+        h = h * (0.9 * orc_pow(err, -1.0 / (double)order));
+    }
+    return h;
+}
+
+int orc_traj_alloc(orc_trajectory *tr, int64_t cap, int n)
+{
+    tr->cap = cap;
+    tr->T = 0;
+    tr->t = (double *)malloc(sizeof(double) * (size_t)cap);
+    tr->h = (double *)calloc((size_t)cap, sizeof(double));
+    tr->y = (double *)malloc(sizeof(double) * (size_t)cap * (size_t)n);
+    return (tr->t && tr->h && tr->y) ? 0 : -1;
+}
+
+void orc_traj_free(orc_trajectory *tr)
+{
+    free(tr->t);
+    free(tr->h);
+    free(tr->y);
+    tr->t = tr->h = tr->y = NULL;
+    tr->cap = tr->T = 0;
+}
+
+static int record(orc_trajectory *tr, int n, double t, const double *y)
+{
+    if (!tr) return 0;
+    if (tr->T >= tr->cap) return -1;
+    tr->t[tr->T] = t;
+    tr->h[tr->T] = 0.0;
+    memcpy(tr->y + (size_t)tr->T * n, y, sizeof(double) * (size_t)n);
+    tr->T += 1;
+    return 0;
+}
+
+static int all_finite(int n, const double *y)
+{
+    for (int i = 0; i < n; ++i)
+        if (!isfinite(y[i])) return 0;
+    return 1;
+}
+
+int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_trajectory *tr,
+                    orc_stats *st, double *log, int64_t log_cap, int64_t *log_len)
+{
+    orc_tableau tb;
+    orc_work w;
+    int n = pb->n;
+    orc_stats s0 = {0, 0, 0, ORC_OK};
+    int64_t nlog = 0;
+    if (tr) tr->T = 0;
+    if (orc_get_tableau(pb->tableau, &tb) != 0 || n <= 0 || n > ORC_MAX_N ||
+        (pb->controlled && !tb.has_err)) {
+        s0.status = ORC_BAD_INPUT;
+        if (st) *st = s0;
+        return 0;
+    }
+    double t = pb->t0, t1 = pb->t1, dt = pb->dt0;
+    int64_t count = 0;
+
+    if (!pb->controlled) {
+        /* detail/runge_kutta.hpp:36-72 (stepper_tag) */
+        int64_t step = 0;
+        while (less_eq_with_sign(t + dt, t1, dt)) {
+            if (record(tr, n, t, y) != 0 || (pb->max_steps > 0 && step >= pb->max_steps)) {
+                s0.status = ORC_MAX_STEPS;
+                goto done;
+            }
+            orc_rhs(pb->system, n, y, p, t, pb->sys_const, w.K[0]);
+            s0.rhs_evals += 1 + attempt(pb, &tb, y, p, t, dt, &w);
+            memcpy(y, w.ynew, sizeof(double) * (size_t)n);
+            if (tr) tr->h[tr->T - 1] = dt;
+            ++step;
+            s0.accepted = (int32_t)step;
+            t = pb->t0 + (double)step * dt;
+        }
+        if (record(tr, n, t, y) != 0) s0.status = ORC_MAX_STEPS;
+        count = step;
+        goto done;
+    }
+
+    /* detail/runge_kutta.hpp:74-118 (controlled_stepper_tag) */
+    int have_k0 = 0;
+    while (less_with_sign(t, t1, dt)) {
+        if (record(tr, n, t, y) != 0 || (pb->max_steps > 0 && count >= pb->max_steps)) {
+            s0.status = ORC_MAX_STEPS;
+            goto done;
+        }
+        if (less_with_sign(t1, t + dt, dt)) dt = t1 - t;
+        int tries = 0;
+        int ok = 0;
+        /* odeint evaluates f(y) on every try of a non-FSAL stepper; the value
+         * does not change between tries, so it is evaluated once per step.  A
+         * FSAL stepper (DOPRI5) evaluates it on its first call only and then
+         * carries f(y_{n+1}) over from the last stage. */
+        if (!tb.fsal || !have_k0) {
+            orc_rhs(pb->system, n, y, p, t, pb->sys_const, w.K[0]);
+            s0.rhs_evals += 1;
+            have_k0 = 1;
+        }
+        do {
+            s0.rhs_evals += attempt(pb, &tb, y, p, t, dt, &w);
+            double err = error_norm(n, pb->abs_tol, pb->rel_tol, dt, y, w.K[0], w.xerr);
+            double t_at = t, h_at = dt;
+            if (err > 1.0) {
+                dt = decrease_step(dt, err, tb.err_order);
+                s0.rejected += 1;
+                ok = 0;
+            } else {
+                if (tr) tr->h[tr->T - 1] = dt;
+                t = t + dt;
+                dt = increase_step(dt, err, tb.order);
+                memcpy(y, w.ynew, sizeof(double) * (size_t)n);
+                if (tb.fsal) memcpy(w.K[0], w.K[tb.s_fwd - 1], sizeof(double) * (size_t)n);
+                ok = 1;
+            }
+            if (log && nlog < log_cap) {
+                log[4 * nlog + 0] = t_at;
+                log[4 * nlog + 1] = h_at;
+                log[4 * nlog + 2] = err;
+                log[4 * nlog + 3] = (double)ok;
+                ++nlog;
+            }
+            ++tries; /* failed_step_checker: throws on the try after 500 */
+            if (tries > pb->max_attempts) {
+                if (ok) s0.accepted = (int32_t)(count + 1);
+                s0.status = ORC_MAX_FAIL;
+                goto done;
+            }
+        } while (!ok);
+        ++count;
+        s0.accepted = (int32_t)count;
+    }
+    if (record(tr, n, t, y) != 0) s0.status = ORC_MAX_STEPS;
+
+done:
+    if (s0.status == ORC_OK && !all_finite(n, y)) s0.status = ORC_NONFINITE;
+    if (st) *st = s0;
+    if (log_len) *log_len = nlog;
+    return count;
+}
+
+void orc_replay(const orc_problem *pb, const double *y0, const double *p,
+                const orc_trajectory *tr, double *yT)
+{
+    orc_tableau tb;
+    orc_work w;
+    int n = pb->n;
+    double y[ORC_MAX_N];
+    orc_get_tableau(pb->tableau, &tb);
+    memcpy(y, y0, sizeof(double) * (size_t)n);
+    for (int64_t k = 0; k + 1 < tr->T; ++k) {
+        double h = tr->h[k];
+        orc_rhs(pb->system, n, y, p, tr->t[k], pb->sys_const, w.K[0]);
+        attempt(pb, &tb, y, p, tr->t[k], h, &w);
+        memcpy(y, w.ynew, sizeof(double) * (size_t)n);
+    }
+    memcpy(yT, y, sizeof(double) * (size_t)n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint, detail/backpropagation.hpp:83-158 (one seed).          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double K[ORC_MAX_STAGES][ORC_MAX_N];
+    double X[ORC_MAX_STAGES][ORC_MAX_N];
+    double Kb[ORC_MAX_STAGES][ORC_MAX_N];
+    double ybar[ORC_MAX_N];
+    double xi[ORC_MAX_N];
+} orc_rwork;
+
+static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const double *p,
+                           const orc_trajectory *tr, int64_t nidx, double *lambda,
+                           double *pbar, double *wjp, orc_rwork *w)
+{
+    int n = pb->n, np = pb->np, s = tb->s;
+    double tn = tr->t[nidx];
+    /* The reference takes h = t[n+1] - t[n] (StateStorage.hpp:22), which can differ
+     * from the accepted step by half an ulp of t; the exact accepted h is stored
+     * instead, so this is the exact derivative of the forward map. */
+    double h = tr->h[nidx];
+    const double *yn = tr->y + (size_t)nidx * n;
+
+    /* compute_intermediate_states (:24-64): recompute K_m from y_n */
+    for (int m = 0; m < s; ++m) {
+        stage_input(tb, m, n, h, yn, w->K, w->X[m]);
+        orc_rhs(pb->system, n, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->K[m]);
+    }
+    /* seed (:104-112): ybar = lambda_{n+1}; Kbar_m = (b_m h) lambda_{n+1} */
+    for (int i = 0; i < n; ++i) w->ybar[i] = lambda[i];
+    for (int m = 0; m < s; ++m) {
+        double bh = tb->b[m] * h;
+        for (int i = 0; i < n; ++i) w->Kb[m][i] = bh * lambda[i];
+    }
+    /* stages s..1 (:126-153) */
+    for (int m = s - 1; m >= 0; --m) {
+        orc_vjp(pb->system, n, np, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->Kb[m], w->xi,
+                wjp);
+        for (int i = 0; i < n; ++i) w->ybar[i] += w->xi[i];
+        for (int j = 0; j < m; ++j) {
+            double a = tb->a[m][j];
+            if (a == 0.0) continue;
+            double ah = a * h;
+            for (int i = 0; i < n; ++i) w->Kb[j][i] = fma(ah, w->xi[i], w->Kb[j][i]);
+        }
+        for (int k = 0; k < np; ++k) pbar[k] += wjp[k];
+    }
+    for (int i = 0; i < n; ++i) lambda[i] = w->ybar[i];
+}
+
+void orc_backward(const orc_problem *pb, const double *p, const orc_trajectory *tr,
+                  double *lambda, double *mu)
+{
+    orc_tableau tb;
+    orc_get_tableau(pb->tableau, &tb);
+    int np = pb->np;
+    orc_rwork *w = (orc_rwork *)malloc(sizeof(orc_rwork));
+    double *pbar = (double *)calloc((size_t)np, sizeof(double));
+    double *wjp = (double *)calloc((size_t)np, sizeof(double));
+    /* back_prop (:231-254): n = T-2 ... 0 */
+    for (int64_t k = tr->T - 2; k >= 0; --k) back_prop_step(pb, &tb, p, tr, k, lambda, pbar, wjp, w);
+    for (int k = 0; k < np; ++k) mu[k] += pbar[k];
+    free(wjp);
+    free(pbar);
+    free(w);
+}
+
+void orc_adjoint_solve(const orc_problem *pb, const double *p, const orc_trajectory *tr,
+                       int nout, double *lambda, double *mu)
+{
+    for (int o = 0; o < nout; ++o)
+        orc_backward(pb, p, tr, lambda + (size_t)o * pb->n, mu + (size_t)o * pb->np);
+}
+
+void orc_sensitivity_matrix(const orc_problem *pb, const double *p, const orc_trajectory *tr,
+                            double *S)
+{
+    int n = pb->n, np = pb->np;
+    double lam[ORC_MAX_N];
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) lam[j] = (i == j) ? 1.0 : 0.0;
+        double *row = S + (size_t)i * np;
+        for (int k = 0; k < np; ++k) row[k] = 0.0;
+        orc_backward(pb, p, tr, lam, row);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Batched driver (SoA in/out), pthreads over instances.                     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const orc_problem *pb;
+    int64_t B, b_begin, b_end;
+    const double *y0, *p, *p_shared;
+    int cost;
+    const double *cost_w;
+    double *J, *yT, *dJdy0, *dJdp;
+    int32_t *n_acc, *n_rej, *status;
+    int64_t next; /* shared work counter */
+    pthread_mutex_t mtx;
+    double *shared_acc;   /* sum of dJ/dA over instances */
+    int np_full;
+} batch_ctx;
+
+static void *batch_worker(void *arg)
+{
+    batch_ctx *c = (batch_ctx *)arg;
+    const orc_problem *pb = c->pb;
+    int n = pb->n;
+    int shared = pb->system == ORC_SYS_LOTKA_VOLTERRA_SHARED;
+    int npf = c->np_full;
+    orc_problem lp = *pb;
+    lp.np = npf;
+    orc_trajectory tr;
+    int64_t cap = (pb->max_steps > 0 ? pb->max_steps : 100000) + 1;
+    if (orc_traj_alloc(&tr, cap, n) != 0) return NULL;
+    double *pp = (double *)malloc(sizeof(double) * (size_t)npf);
+    double *mu = (double *)malloc(sizeof(double) * (size_t)npf);
+    double *acc = shared ? (double *)calloc((size_t)n * n, sizeof(double)) : NULL;
+    double y[ORC_MAX_N], lam[ORC_MAX_N];
+    const int64_t chunk = 8;
+    for (;;) {
+        int64_t b0 = __atomic_fetch_add(&c->next, chunk, __ATOMIC_RELAXED);
+        if (b0 >= c->b_end) break;
+        int64_t b1 = b0 + chunk < c->b_end ? b0 + chunk : c->b_end;
+        for (int64_t b = b0; b < b1; ++b) {
+            int64_t B = c->B;
+            for (int i = 0; i < n; ++i) y[i] = c->y0[(size_t)i * B + b];
+            if (shared) {
+                for (int i = 0; i < n; ++i) pp[i] = c->p[(size_t)i * B + b];
+                memcpy(pp + n, c->p_shared, sizeof(double) * (size_t)n * n);
+            } else {
+                for (int k = 0; k < npf; ++k) pp[k] = c->p[(size_t)k * B + b];
+            }
+            orc_stats st;
+            orc_forward(&lp, y, pp, &tr, &st, NULL, 0, NULL);
+            if (c->n_acc) c->n_acc[b] = st.accepted;
+            if (c->n_rej) c->n_rej[b] = st.rejected;
+            if (c->status) c->status[b] = st.status;
+            int okf = st.status == ORC_OK;
+            double Jv = 0.0;
+            for (int i = 0; i < n; ++i) {
+                switch (c->cost) {
+                case ORC_COST_SUM: Jv += y[i]; lam[i] = 1.0; break;
+                case ORC_COST_SQUARED_NORM: Jv = fma(y[i], y[i], Jv); lam[i] = 2.0 * y[i]; break;
+                default: Jv = fma(c->cost_w[i], y[i], Jv); lam[i] = c->cost_w[i]; break;
+                }
+            }
+            if (c->J) c->J[b] = okf ? Jv : NAN;
+            if (c->yT)
+                for (int i = 0; i < n; ++i) c->yT[(size_t)i * B + b] = y[i];
+            for (int k = 0; k < npf; ++k) mu[k] = 0.0;
+            if (okf) orc_backward(&lp, pp, &tr, lam, mu);
+            if (c->dJdy0)
+                for (int i = 0; i < n; ++i) c->dJdy0[(size_t)i * B + b] = okf ? lam[i] : NAN;
+            if (c->dJdp) {
+                int npi = shared ? n : npf;
+                for (int k = 0; k < npi; ++k) c->dJdp[(size_t)k * B + b] = okf ? mu[k] : NAN;
+            }
+            if (shared && okf)
+                for (int k = 0; k < n * n; ++k) acc[k] += mu[n + k];
+        }
+    }
+    if (shared && c->shared_acc) {
+        pthread_mutex_lock(&c->mtx);
+        for (int k = 0; k < n * n; ++k) c->shared_acc[k] += acc[k];
+        pthread_mutex_unlock(&c->mtx);
+    }
+    free(acc);
+    free(mu);
+    free(pp);
+    orc_traj_free(&tr);
+    return NULL;
+}
+
+int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b_count,
+                    const double *y0, const double *p, const double *p_shared,
+                    int cost, const double *cost_w,
+                    double *J, double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
+                    int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads)
+{
+    if (!pb || pb->n <= 0 || pb->n > ORC_MAX_N || b_begin < 0 || b_begin + b_count > B) return -1;
+    if (cost == ORC_COST_LINEAR && !cost_w) return -1;
+    int shared = pb->system == ORC_SYS_LOTKA_VOLTERRA_SHARED;
+    if (shared && !p_shared) return -1;
+    batch_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.pb = pb;
+    c.B = B;
+    c.b_begin = b_begin;
+    c.b_end = b_begin + b_count;
+    c.y0 = y0;
+    c.p = p;
+    c.p_shared = p_shared;
+    c.cost = cost;
+    c.cost_w = cost_w;
+    c.J = J;
+    c.yT = yT;
+    c.dJdy0 = dJdy0;
+    c.dJdp = dJdp;
+    c.n_acc = n_acc;
+    c.n_rej = n_rej;
+    c.status = status;
+    c.next = b_begin;
+    c.np_full = shared ? pb->n + pb->n * pb->n : pb->np;
+    pthread_mutex_init(&c.mtx, NULL);
+    double *acc = NULL;
+    if (shared) {
+        acc = (double *)calloc((size_t)pb->n * pb->n, sizeof(double));
+        c.shared_acc = acc;
+    }
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    int started = 0;
+    for (int i = 1; i < nthreads; ++i)
+        if (pthread_create(&th[i], NULL, batch_worker, &c) == 0) ++started;
+        else break;
+    batch_worker(&c);
+    for (int i = 1; i <= started; ++i) pthread_join(th[i], NULL);
+    pthread_mutex_destroy(&c.mtx);
+    if (shared) {
+        if (dJdp_shared)
+            for (int k = 0; k < pb->n * pb->n; ++k) dJdp_shared[k] = acc[k];
+        free(acc);
+    }
+    return 0;
+}

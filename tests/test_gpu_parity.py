@@ -1,0 +1,95 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same
+seeded inputs.
+
+Bar (per instance): identical accepted/rejected step counts; y(T) equal to the
+oracle (the forward pass follows the oracle's IEEE operation sequence, so it
+is compared bit for bit); J, dJ/dy0, dJ/dp within a norm-wise relative error
+of 1e-10 (the stated floating-point tolerance).  The persistent one-lane
+kernels also follow the oracle's reverse sequence, so their gradients are
+checked at 1e-13 as well.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "no HIP device"
+    return torch
+
+
+def _dev(a, torch):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _solve_gpu(w, inputs, mode, **kw):
+    torch = _torch()
+    import vectorizedadjoint_amd as va
+    B = inputs["y0"].shape[1]
+    pb = va.Problem.from_workload(w, batch=B, mode=mode, **kw)
+    print(f"[gpu] solve {w.name} B={B} mode={mode} {kw}", flush=True)
+    with va.Solver(pb) as s:
+        y0 = _dev(inputs["y0"], torch)
+        p = _dev(inputs["p"], torch)
+        out = s.solve(y0, p, w.cost)
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def _compare(g, o, bitwise_fwd=True, grad_tol=TOL):
+    ok = o["status"] == 0
+    assert np.array_equal(g["status"], o["status"])
+    same_steps = (g["n_acc"] == o["n_acc"]) & (g["n_rej"] == o["n_rej"])
+    frac = same_steps[ok].mean()
+    assert frac >= 0.999, f"accept/reject sequences equal on only {frac:.5f} of instances"
+    m = ok & same_steps
+    if bitwise_fwd:
+        assert np.array_equal(g["yT"][:, m], o["yT"][:, m]), "y(T) differs from the oracle"
+    e_y = rel_err(g["yT"][:, m], o["yT"][:, m]).max()
+    e_J = rel_err(g["J"][m], o["J"][m]).max()
+    e_y0 = rel_err(g["dJdy0"][:, m], o["dJdy0"][:, m]).max()
+    e_p = rel_err(g["dJdp"][:, m], o["dJdp"][:, m]).max()
+    print(f"    same steps {frac:.6f}  max rel err: yT {e_y:.2e} J {e_J:.2e} "
+          f"dJdy0 {e_y0:.2e} dJdp {e_p:.2e}", flush=True)
+    assert e_y <= TOL and e_J <= TOL
+    assert e_y0 <= grad_tol and e_p <= grad_tol
+    assert np.all(np.isnan(g["dJdp"][:, ~ok]))
+
+
+CASES = [("vdp", 1024), ("oscillator", 4096), ("glv10", 2048)]
+
+
+@pytest.mark.parametrize("name,B", CASES)
+def test_persistent_matches_oracle(name, B):
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT
+    w = I.WORKLOADS[name]
+    inp = I.make_inputs(w, B)
+    o = run_oracle(w, inp, max_steps=4096)
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096)
+    _compare(g, o, grad_tol=1e-13)
+
+
+@pytest.mark.parametrize("name,B", CASES)
+def test_plain_matches_oracle(name, B):
+    from vectorizedadjoint_amd import inputs as I, MODE_PLAIN
+    w = I.WORKLOADS[name]
+    inp = I.make_inputs(w, B)
+    o = run_oracle(w, inp, max_steps=2048)
+    g = _solve_gpu(w, inp, MODE_PLAIN, max_steps=2048)
+    _compare(g, o)
+
+
+def test_refill_with_few_lanes_matches_oracle():
+    """Fewer lanes than instances: every lane solves many instances in turn."""
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT
+    w = I.WORKLOADS["glv10"]
+    inp = I.make_inputs(w, 1500)
+    o = run_oracle(w, inp, max_steps=1024)
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, lanes=128, block_size=64)
+    _compare(g, o, grad_tol=1e-13)

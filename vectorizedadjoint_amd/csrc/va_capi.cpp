@@ -1,0 +1,512 @@
+// va_capi.cpp -- implementation of the C ABI declared in include/vectorizedadjoint_amd.h.
+//
+// A handle mirrors the reference's Driver (lib/include/Driver.hpp:15-79): it
+// owns the trajectory store and the stepper/system choice; the caller owns
+// every input and output buffer.  All work is stream-ordered on the caller's
+// stream; nothing here synchronises except va_max_steps_seen and the plain
+// form's completion polling.
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "../../include/vectorizedadjoint_amd.h"
+#include "va_kernels.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+va_status fail(va_status s, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return s;
+}
+
+#define VA_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(VA_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),    \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, double v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_f64(double *dst, const double *src, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_copy_i32(int32_t *dst, const int32_t *src, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_identity_seeds(double *lambda, int n, int64_t B)
+{
+    // lambda[o][i][b] = (o == i)
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)n * n * B;
+    if (idx >= total) return;
+    const int64_t oi = idx / B;
+    const int o = (int)(oi / n), i = (int)(oi % n);
+    lambda[idx] = (o == i) ? 1.0 : 0.0;
+}
+
+inline int blocks_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+}  // namespace
+
+struct va_handle {
+    va_problem pb;
+    va::KernelSet ks;
+    int device = 0;
+    int cu_count = 0;
+    int block = 256;
+    int64_t lanes = 0;
+    // persistent store
+    double *traj = nullptr;
+    // plain store + state
+    double *ptraj = nullptr, *pl_t = nullptr, *pl_dt = nullptr, *pl_k0 = nullptr,
+           *pl_pbar = nullptr;
+    int32_t *pl_done = nullptr;
+    // bookkeeping [B] and counters
+    int32_t *n_acc = nullptr, *n_rej = nullptr, *status = nullptr;
+    int32_t *counters = nullptr;  // [0] fwd queue, [1] rev queue, [2] max steps, [3] active
+    int32_t *host_counter = nullptr;
+    bool have_forward = false;
+};
+
+namespace {
+
+va_status fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
+}
+
+va_status fill_f64(double *p, double v, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_f64, dim3(blocks_for(n, 256)), dim3(256), 0, s, p, v, n);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
+}
+
+va::SolveParams solve_params(const va_handle *h)
+{
+    va::SolveParams sp;
+    sp.B = h->pb.batch;
+    sp.cap = h->pb.max_steps;
+    sp.t0 = h->pb.t0;
+    sp.t1 = h->pb.t1;
+    sp.dt0 = h->pb.dt0;
+    sp.atol = h->pb.abs_tol;
+    sp.rtol = h->pb.rel_tol;
+    sp.kconst = h->pb.system_constant;
+    sp.max_attempts = h->pb.max_attempts;
+    sp.adaptive = h->pb.adaptive;
+    return sp;
+}
+
+va_status check_handle(const va_handle *h)
+{
+    if (!h) return fail(VA_ERR_INVALID, "null handle");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != h->device)
+        return fail(VA_ERR_INVALID, "handle belongs to device %d, current device is %d", h->device,
+                    dev);
+    return VA_OK;
+}
+
+void release(va_handle *h)
+{
+    if (!h) return;
+    hipFree(h->traj);
+    hipFree(h->ptraj);
+    hipFree(h->pl_t);
+    hipFree(h->pl_dt);
+    hipFree(h->pl_k0);
+    hipFree(h->pl_pbar);
+    hipFree(h->pl_done);
+    hipFree(h->n_acc);
+    hipFree(h->n_rej);
+    hipFree(h->status);
+    hipFree(h->counters);
+    if (h->host_counter) hipHostFree(h->host_counter);
+}
+
+// ---------------- persistent form ----------------
+va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st = fill_i32(h->counters, 0, 4, s);
+    if (st) return st;
+    va::FwdArgs a;
+    a.sp = solve_params(h);
+    a.y = y;
+    a.p = p;
+    a.traj = h->traj;
+    a.n_acc = h->n_acc;
+    a.n_rej = h->n_rej;
+    a.status = h->status;
+    a.counter = h->counters + 0;
+    a.max_steps_seen = h->counters + 2;
+    const int64_t lanes = std::min<int64_t>(h->lanes, B);
+    VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->block), h->block, s));
+    return VA_OK;
+}
+
+va_status reverse_persistent(va_handle *h, const double *p, int cost, const double *cost_w,
+                             const double *yT, double *J, double *lambda, double *mu, int nout,
+                             hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st = fill_i32(h->counters + 1, 0, 1, s);
+    if (st) return st;
+    va::RevArgs a;
+    a.sp = solve_params(h);
+    a.p = p;
+    a.traj = h->traj;
+    a.n_acc = h->n_acc;
+    a.status = h->status;
+    a.cost = cost;
+    a.cost_w = cost_w;
+    a.yT = yT;
+    a.J = J;
+    a.lambda = lambda;
+    a.mu = mu;
+    a.nout = nout;
+    a.counter = h->counters + 1;
+    const int64_t lanes = std::min<int64_t>(h->lanes, B);
+    VA_HIP(h->ks.rev(a, blocks_for(lanes, h->block), h->block, s));
+    return VA_OK;
+}
+
+// ---------------- plain form ----------------
+va::PlainState plain_state(va_handle *h)
+{
+    va::PlainState ps;
+    ps.t = h->pl_t;
+    ps.dt = h->pl_dt;
+    ps.k0 = h->pl_k0;
+    ps.done = h->pl_done;
+    ps.active = h->counters + 3;
+    ps.max_steps_seen = h->counters + 2;
+    ps.pbar = h->pl_pbar;
+    return ps;
+}
+
+va_status forward_plain(va_handle *h, double *y, const double *p, hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st;
+    if ((st = fill_i32(h->pl_done, 0, B, s))) return st;
+    if ((st = fill_i32(h->counters + 2, 0, 1, s))) return st;
+    if ((st = fill_i32(h->counters + 3, (int32_t)B, 1, s))) return st;
+    va::PlainFwdArgs a;
+    a.sp = solve_params(h);
+    a.y = y;
+    a.p = p;
+    a.traj = h->ptraj;
+    a.n_acc = h->n_acc;
+    a.n_rej = h->n_rej;
+    a.status = h->status;
+    a.st = plain_state(h);
+    const int grid = blocks_for(B, h->block);
+    const int32_t last = h->pb.max_steps + 1;
+    const int chunk = 8;
+    for (int32_t step = 0; step <= last; step += chunk) {
+        for (int32_t k = step; k < std::min(step + chunk, last + 1); ++k) {
+            a.step = k;
+            VA_HIP(h->ks.plain_fwd(a, grid, h->block, s));
+        }
+        VA_HIP(hipMemcpyAsync(h->host_counter, h->counters + 3, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, s));
+        VA_HIP(hipStreamSynchronize(s));
+        if (*h->host_counter <= 0) break;
+    }
+    return VA_OK;
+}
+
+va_status read_max_steps(va_handle *h, int32_t *out, hipStream_t s)
+{
+    VA_HIP(hipMemcpyAsync(h->host_counter, h->counters + 2, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
+    VA_HIP(hipStreamSynchronize(s));
+    *out = *h->host_counter;
+    return VA_OK;
+}
+
+// One seed: lambda [N][B] in/out, mu [P][B] +=.
+va_status reverse_plain_seed(va_handle *h, const double *p, double *lambda, double *mu,
+                             int32_t smax, hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st;
+    if ((st = fill_f64(h->pl_pbar, 0.0, (int64_t)h->ks.np * B, s))) return st;
+    va::PlainRevArgs a;
+    a.sp = solve_params(h);
+    a.p = p;
+    a.traj = h->ptraj;
+    a.n_acc = h->n_acc;
+    a.status = h->status;
+    a.lambda = lambda;
+    a.st = plain_state(h);
+    const int grid = blocks_for(B, h->block);
+    for (int32_t k = 0; k < smax; ++k) {
+        a.k = k;
+        VA_HIP(h->ks.plain_rev(a, grid, h->block, s));
+    }
+    VA_HIP(va::launch_plain_finish(lambda, lambda, mu, h->pl_pbar, h->status, B, h->ks.n,
+                                   h->ks.np, grid, h->block, s));
+    return VA_OK;
+}
+
+va_status copy_out(va_handle *h, int32_t *n_acc, int32_t *n_rej, int32_t *lane_status,
+                   hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    const int g = blocks_for(B, 256);
+    if (n_acc) hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, n_acc, h->n_acc, B);
+    if (n_rej) hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, n_rej, h->n_rej, B);
+    if (lane_status)
+        hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, lane_status, h->status, B);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void va_problem_init(va_problem *pb)
+{
+    if (!pb) return;
+    memset(pb, 0, sizeof(*pb));
+    pb->abi_version = VA_ABI_VERSION;
+    pb->adaptive = 1;
+    pb->dt0 = 1e-3;
+    pb->abs_tol = 1e-6;
+    pb->rel_tol = 1e-6;
+    pb->max_attempts = 500;
+    pb->max_steps = 4096;
+    pb->mode = VA_MODE_PERSISTENT;
+}
+
+const char *va_last_error(void) { return g_err; }
+
+va_status va_create(const va_problem *pb, va_handle **out)
+{
+    g_err[0] = 0;
+    if (!pb || !out) return fail(VA_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (pb->abi_version != VA_ABI_VERSION)
+        return fail(VA_ERR_INVALID, "abi_version %d, library implements %d", pb->abi_version,
+                    VA_ABI_VERSION);
+    if (pb->batch <= 0 || pb->batch > (int64_t)0x7fffffff)
+        return fail(VA_ERR_INVALID, "batch must be in [1, 2^31), got %lld", (long long)pb->batch);
+    if (pb->max_steps < 1 || pb->max_attempts < 1)
+        return fail(VA_ERR_INVALID, "max_steps and max_attempts must be >= 1");
+    if (!(pb->dt0 > 0.0) || !(pb->t1 >= pb->t0))
+        return fail(VA_ERR_INVALID, "need dt0 > 0 and t1 >= t0");
+    if (pb->adaptive && !(pb->abs_tol >= 0.0 && pb->rel_tol >= 0.0))
+        return fail(VA_ERR_INVALID, "tolerances must be >= 0");
+    if (pb->mode != VA_MODE_PERSISTENT && pb->mode != VA_MODE_PLAIN)
+        return fail(VA_ERR_INVALID, "unknown mode %d", pb->mode);
+    if (pb->adaptive &&
+        (pb->tableau == VA_TABLEAU_EULER || pb->tableau == VA_TABLEAU_RK4))
+        return fail(VA_ERR_INVALID,
+                    "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
+    va::KernelSet ks;
+    if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive, &ks))
+        return fail(VA_ERR_UNSUPPORTED, "no kernels for system %d, N=%d, tableau %d, adaptive %d",
+                    pb->system, pb->n_state, pb->tableau, pb->adaptive);
+    if (pb->n_param != ks.np)
+        return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
+                    pb->system, pb->n_state, ks.np, pb->n_param);
+
+    va_handle *h = new (std::nothrow) va_handle();
+    if (!h) return fail(VA_ERR_OUT_OF_MEMORY, "host allocation");
+    h->pb = *pb;
+    h->ks = ks;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, h->device) != hipSuccess) {
+        delete h;
+        return fail(VA_ERR_HIP, "no HIP device");
+    }
+    h->cu_count = prop.multiProcessorCount;
+    h->block = pb->block_size > 0 ? pb->block_size : 256;
+    if (h->block % 64 != 0 || h->block > 256) {
+        delete h;
+        return fail(VA_ERR_INVALID, "block_size must be a multiple of 64 and <= 256");
+    }
+    const int64_t B = pb->batch;
+    h->lanes = pb->lanes > 0 ? pb->lanes : B;
+
+    const size_t rows = (size_t)pb->max_steps + 1;
+    hipError_t e = hipSuccess;
+    auto alloc = [&](void **ptr, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(ptr, bytes);
+    };
+    alloc((void **)&h->n_acc, sizeof(int32_t) * B);
+    alloc((void **)&h->n_rej, sizeof(int32_t) * B);
+    alloc((void **)&h->status, sizeof(int32_t) * B);
+    alloc((void **)&h->counters, sizeof(int32_t) * 4);
+    if (pb->mode == VA_MODE_PERSISTENT) {
+        alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
+    } else {
+        const size_t F = 2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n;
+        alloc((void **)&h->ptraj, sizeof(double) * rows * F * (size_t)B);
+        alloc((void **)&h->pl_t, sizeof(double) * B);
+        alloc((void **)&h->pl_dt, sizeof(double) * B);
+        alloc((void **)&h->pl_k0, sizeof(double) * B * ks.n);
+        alloc((void **)&h->pl_pbar, sizeof(double) * B * ks.np);
+        alloc((void **)&h->pl_done, sizeof(int32_t) * B);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&h->host_counter, sizeof(int32_t));
+    if (e != hipSuccess) {
+        release(h);
+        delete h;
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? VA_ERR_OUT_OF_MEMORY : VA_ERR_HIP,
+                    "allocating the trajectory store (%zu rows x %lld instances): %s", rows,
+                    (long long)B, hipGetErrorString(e));
+    }
+    *out = h;
+    return VA_OK;
+}
+
+void va_destroy(va_handle *h)
+{
+    if (!h) return;
+    release(h);
+    delete h;
+}
+
+va_status va_forward(va_handle *h, double *y, const double *p, int32_t *n_accepted,
+                     int32_t *n_rejected, int32_t *lane_status, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!y || !p) return fail(VA_ERR_INVALID, "y and p are required");
+    hipStream_t s = (hipStream_t)stream;
+    st = h->pb.mode == VA_MODE_PLAIN ? forward_plain(h, y, p, s) : forward_persistent(h, y, p, s);
+    if (st) return st;
+    h->have_forward = true;
+    return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
+va_status va_adjoint(va_handle *h, const double *p, int32_t nout, double *lambda, double *mu,
+                     void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!h->have_forward) return fail(VA_ERR_STATE, "va_adjoint needs a va_forward first");
+    if (!p || !lambda || !mu || nout < 1)
+        return fail(VA_ERR_INVALID, "p, lambda, mu required and nout >= 1");
+    hipStream_t s = (hipStream_t)stream;
+    if (h->pb.mode == VA_MODE_PLAIN) {
+        int32_t smax = 0;
+        if ((st = read_max_steps(h, &smax, s))) return st;
+        const int64_t B = h->pb.batch;
+        for (int o = 0; o < nout; ++o) {
+            st = reverse_plain_seed(h, p, lambda + (size_t)o * h->ks.n * B,
+                                    mu + (size_t)o * h->ks.np * B, smax, s);
+            if (st) return st;
+        }
+        return VA_OK;
+    }
+    return reverse_persistent(h, p, 0, nullptr, nullptr, nullptr, lambda, mu, nout, s);
+}
+
+va_status va_sensitivity_matrix(va_handle *h, const double *p, double *S, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!p || !S) return fail(VA_ERR_INVALID, "p and S are required");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = h->pb.batch;
+    const int n = h->ks.n, np = h->ks.np;
+    double *lam = nullptr;
+    VA_HIP(hipMallocAsync((void **)&lam, sizeof(double) * (size_t)n * n * B, s));
+    hipLaunchKernelGGL(k_identity_seeds, dim3(blocks_for((int64_t)n * n * B, 256)), dim3(256), 0,
+                       s, lam, n, B);
+    if ((st = fill_f64(S, 0.0, (int64_t)n * np * B, s))) {
+        hipFreeAsync(lam, s);
+        return st;
+    }
+    st = va_adjoint(h, p, n, lam, S, stream);
+    hipFreeAsync(lam, s);
+    return st;
+}
+
+va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost,
+                   const double *cost_weights, double *J, double *yT, double *dJdy0,
+                   double *dJdp, int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
+                   void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!y0 || !p || !J || !yT || !dJdy0 || !dJdp)
+        return fail(VA_ERR_INVALID, "y0, p, J, yT, dJdy0, dJdp are required");
+    if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
+        return fail(VA_ERR_INVALID, "unknown cost %d", cost);
+    if (cost == VA_COST_LINEAR && !cost_weights)
+        return fail(VA_ERR_INVALID, "VA_COST_LINEAR needs cost_weights");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = h->pb.batch;
+    const int n = h->ks.n, np = h->ks.np;
+    if (yT != y0) {
+        hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)n * B, 256)), dim3(256), 0, s, yT,
+                           y0, (int64_t)n * B);
+        VA_HIP(hipGetLastError());
+    }
+    if (h->pb.mode == VA_MODE_PLAIN) {
+        if ((st = forward_plain(h, yT, p, s))) return st;
+        h->have_forward = true;
+        VA_HIP(va::launch_plain_cost(cost, cost_weights, yT, J, dJdy0, h->status, B, n,
+                                     blocks_for(B, h->block), h->block, s));
+        if ((st = fill_f64(dJdp, 0.0, (int64_t)np * B, s))) return st;
+        int32_t smax = 0;
+        if ((st = read_max_steps(h, &smax, s))) return st;
+        if ((st = reverse_plain_seed(h, p, dJdy0, dJdp, smax, s))) return st;
+    } else {
+        if ((st = forward_persistent(h, yT, p, s))) return st;
+        h->have_forward = true;
+        if ((st = reverse_persistent(h, p, cost, cost_weights, yT, J, dJdy0, dJdp, 1, s)))
+            return st;
+    }
+    return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
+va_status va_max_steps_seen(va_handle *h, int32_t *out, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!out) return fail(VA_ERR_INVALID, "null out");
+    return read_max_steps(h, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

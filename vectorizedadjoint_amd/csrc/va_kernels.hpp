@@ -1,0 +1,98 @@
+// va_kernels.hpp -- host-side launch interface of the HIP kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace va {
+
+struct SolveParams {
+    int64_t B;
+    int64_t cap;          // accepted-step capacity per instance
+    double t0, t1, dt0, atol, rtol, kconst;
+    int32_t max_attempts;
+    int32_t adaptive;
+};
+
+// Persistent forward: each lane takes instances from *counter until B is reached.
+struct FwdArgs {
+    SolveParams sp;
+    double *y;             // [N][B] in y0, out y(T)
+    const double *p;       // [P][B]
+    double *traj;          // [B][cap+1][N+1] rows (t, y)
+    int32_t *n_acc, *n_rej, *status;
+    int32_t *counter;
+    int32_t *max_steps_seen;
+};
+
+// Persistent reverse over the stored trajectory.
+//   cost > 0: lambda computed from y(T) (read from yT) and written to dJdy0,
+//             dJdp = sum p_bar, J written.
+//   cost == 0: lambda [nout][N][B] in/out, mu [nout][P][B] += (reference semantics).
+struct RevArgs {
+    SolveParams sp;
+    const double *p;
+    const double *traj;
+    const int32_t *n_acc, *status;
+    int32_t cost;
+    const double *cost_w;  // [N] for the linear cost
+    const double *yT;      // [N][B]
+    double *J;
+    double *lambda;        // [nout][N][B]
+    double *mu;            // [nout][P][B]
+    int32_t nout;
+    int32_t *counter;
+};
+
+// Plain form: one launch per accepted step, state in HBM, no refill, full
+// trajectory store [step][field][B] with fields (t, y[N], K[S][N]).
+struct PlainState {
+    double *t, *dt;        // [B]
+    double *k0;            // [N][B] f(y_n) carried for FSAL tableaus
+    int32_t *done;         // [B]
+    int32_t *active;       // 1 counter
+    int32_t *max_steps_seen;
+    double *pbar;          // [P][B] reverse accumulator
+};
+
+struct PlainFwdArgs {
+    SolveParams sp;
+    double *y;
+    const double *p;
+    double *traj;
+    int32_t *n_acc, *n_rej, *status;
+    PlainState st;
+    int32_t step;
+};
+
+struct PlainRevArgs {
+    SolveParams sp;
+    const double *p;
+    const double *traj;
+    const int32_t *n_acc, *status;
+    double *lambda;        // [N][B] seed in/out
+    PlainState st;
+    int32_t k;             // reverse launch index
+};
+
+struct KernelSet {
+    int n, np, s_rev, s_fwd;
+    int flop_rhs, flop_vjp;
+    hipError_t (*fwd)(const FwdArgs &, int grid, int block, hipStream_t);
+    hipError_t (*rev)(const RevArgs &, int grid, int block, hipStream_t);
+    hipError_t (*plain_fwd)(const PlainFwdArgs &, int grid, int block, hipStream_t);
+    hipError_t (*plain_rev)(const PlainRevArgs &, int grid, int block, hipStream_t);
+    int fwd_regs, rev_regs;  // informational
+};
+
+// Returns false when (system, n, tableau, adaptive) has no compiled kernels.
+bool get_kernels(int system, int n, int tableau, int adaptive, KernelSet *out);
+
+hipError_t launch_plain_finish(double *lambda_out, const double *lambda_in, double *mu,
+                               const double *pbar, const int32_t *status, int64_t B, int n, int np,
+                               int grid, int block, hipStream_t s);
+hipError_t launch_plain_cost(int cost, const double *cost_w, const double *yT, double *J,
+                             double *lambda, const int32_t *status, int64_t B, int n, int grid,
+                             int block, hipStream_t s);
+
+}  // namespace va
