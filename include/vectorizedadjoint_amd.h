@@ -99,8 +99,14 @@ typedef struct va_problem {
     int32_t max_attempts;      /* tries per step before VA_LANE_MAX_FAIL (odeint: 500) */
     int32_t max_steps;         /* accepted-step capacity of each stored trajectory    */
     int32_t mode;              /* va_mode                                             */
-    int32_t block_size;        /* threads per block, 0 = tuned                        */
-    int64_t lanes;             /* persistent lanes in flight, 0 = tuned               */
+    int32_t block_size;        /* threads per block (64..256, multiple of 64), 0 = tuned */
+    int32_t instances_per_wave;  /* 64: one lane per instance; 64/N: one group of N
+                                    lanes per instance (Lotka-Volterra); 0 = tuned    */
+    int32_t checkpoint_interval; /* reserved, must be 0                               */
+    int64_t lanes;             /* instances in flight in the persistent kernels, 0 = all
+                                  resident slots; fewer than B means lanes are refilled */
+    int64_t trajectory_budget_bytes; /* 0 = unlimited; va_create fails if the store
+                                        would exceed it                               */
 } va_problem;
 
 typedef struct va_handle va_handle;

@@ -109,6 +109,30 @@ def lib():
     return _lib
 
 
+class use_library:
+    """Context manager: temporarily bind this module to another build of
+    va_oracle.c (used by the mutation tests)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self._saved = _lib
+        _lib = None
+        saved_path = globals()["LIB_PATH"]
+        globals()["LIB_PATH"] = self.path
+        try:
+            lib()
+        finally:
+            globals()["LIB_PATH"] = saved_path
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+
+
 def _dp(a):
     if a is None:
         return None

@@ -7,6 +7,13 @@
 #define _GNU_SOURCE
 #include "va_oracle.h"
 
+/* ORC_MUTANT selects a deliberately wrong variant for the mutation tests
+ * (tests/test_oracle_mutants.py builds them into a temporary directory);
+ * 0, the only value liboracle.so is built with, is the restatement itself. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
@@ -259,7 +266,11 @@ void orc_vjp(int system, int n, int np, const double *x, const double *p, double
         double x0 = x[0], x1 = x[1];
         double wm = w[1] * p[0];
         double q = 1.0 - x0 * x0;
+#if ORC_MUTANT == 3 /* planted sign error, test builds only */
+        wJx[0] = wm * fma(2.0 * x0, x1, -1.0);
+#else
         wJx[0] = wm * fma(-2.0 * x0, x1, -1.0);
+#endif
         wJx[1] = fma(wm, q, w[0]);
         wJp[0] = w[1] * fma(q, x1, -x0);
         break;
@@ -286,7 +297,11 @@ void orc_vjp(int system, int n, int np, const double *x, const double *p, double
             double g = p[j];
             for (int k = 0; k < n; ++k) g = fma(Aj[k], x[k], g);
             double acc = w[j] * g;
+#if ORC_MUTANT == 1 /* planted transposed operand, test builds only */
+            for (int i = 0; i < n; ++i) acc = fma(u[i], p[(size_t)n * (j + 1) + i], acc);
+#else
             for (int i = 0; i < n; ++i) acc = fma(u[i], p[(size_t)n * (i + 1) + j], acc);
+#endif
             wJx[j] = acc;
         }
         for (int i = 0; i < n; ++i) {
@@ -325,10 +340,18 @@ static void stage_input(const orc_tableau *tb, int m, int n, double h, const dou
 {
     for (int i = 0; i < n; ++i) X[i] = y[i];
     for (int j = 0; j < m; ++j) {
+#if ORC_MUTANT == 4 /* planted off-by-one stage index, test builds only */
+        double a = tb->a[m][j];
+        int jj = j > 0 ? j - 1 : j;
+        if (a == 0.0) continue;
+        double ah = a * h;
+        for (int i = 0; i < n; ++i) X[i] = fma(ah, K[jj][i], X[i]);
+#else
         double a = tb->a[m][j];
         if (a == 0.0) continue;
         double ah = a * h;
         for (int i = 0; i < n; ++i) X[i] = fma(ah, K[j][i], X[i]);
+#endif
     }
 }
 
@@ -401,7 +424,11 @@ static double error_norm(int n, double atol, double rtol, double h, const double
 /* default_step_adjuster (max_dt = 0) */
 static double decrease_step(double h, double err, int err_order)
 {
+#if ORC_MUTANT == 5 /* planted wrong exponent, test builds only */
+    double f = 0.9 * orc_pow(err, -1.0 / (double)err_order);
+#else
     double f = 0.9 * orc_pow(err, -1.0 / (double)(err_order - 1));
+#endif
     f = (f < 0.2) ? 0.2 : f;
     return h * f;
 }
@@ -621,7 +648,11 @@ static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const d
             double a = tb->a[m][j];
             if (a == 0.0) continue;
             double ah = a * h;
+#if ORC_MUTANT == 2 /* planted dropped term, test builds only */
+            (void)ah;
+#else
             for (int i = 0; i < n; ++i) w->Kb[j][i] = fma(ah, w->xi[i], w->Kb[j][i]);
+#endif
         }
         for (int k = 0; k < np; ++k) pbar[k] += wjp[k];
     }

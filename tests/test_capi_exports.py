@@ -27,7 +27,9 @@ def test_library_exports_every_declared_symbol():
 
 def test_problem_struct_layout_matches_header():
     import vectorizedadjoint_amd as va
-    # int32 x6, int64, double x6, int32 x4, int64 -> 104 bytes
-    assert ctypes.sizeof(va.VaProblem) == 104
+    # int32 x6, int64, double x6, int32 x6, int64 x2 -> 120 bytes
+    assert ctypes.sizeof(va.VaProblem) == 120
     assert va.VaProblem.batch.offset == 24 and va.VaProblem.max_attempts.offset == 80
-    assert va.VaProblem.lanes.offset == 96
+    assert va.VaProblem.instances_per_wave.offset == 96
+    assert va.VaProblem.lanes.offset == 104
+    assert va.VaProblem.trajectory_budget_bytes.offset == 112

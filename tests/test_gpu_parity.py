@@ -85,11 +85,39 @@ def test_plain_matches_oracle(name, B):
     _compare(g, o)
 
 
-def test_refill_with_few_lanes_matches_oracle():
-    """Fewer lanes than instances: every lane solves many instances in turn."""
+@pytest.mark.parametrize("ipw", [64, 6])
+def test_refill_with_few_lanes_matches_oracle(ipw):
+    """Fewer instance slots than instances: every slot solves many instances in turn."""
     from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT
     w = I.WORKLOADS["glv10"]
     inp = I.make_inputs(w, 1500)
     o = run_oracle(w, inp, max_steps=1024)
-    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, lanes=128, block_size=64)
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, lanes=120, block_size=64,
+                   instances_per_wave=ipw)
+    _compare(g, o, grad_tol=1e-13)
+
+
+@pytest.mark.parametrize("ipw", [64, 6])
+@pytest.mark.parametrize("tableau", ["dopri5", "fehlberg78"])
+def test_glv_layouts_and_tableaus_match_oracle(ipw, tableau):
+    """Both Lotka-Volterra layouts (one lane per instance, one 10-lane group per
+    instance) with the other adaptive tableaus, bit-identical forward."""
+    import dataclasses
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT, TAB_DOPRI5, TAB_FEHLBERG78
+    tid = {"dopri5": TAB_DOPRI5, "fehlberg78": TAB_FEHLBERG78}[tableau]
+    w = dataclasses.replace(I.WORKLOADS["glv10"], tableau=tid, tol=1e-7)
+    inp = I.make_inputs(w, 600)
+    o = run_oracle(w, inp, max_steps=1024)
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=ipw)
+    _compare(g, o, grad_tol=1e-13)
+
+
+def test_glv_fixed_step_rk4_rows_matches_oracle():
+    import dataclasses
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT, TAB_RK4
+    w = dataclasses.replace(I.WORKLOADS["glv10"], tableau=TAB_RK4, controlled=False, dt0=0.05,
+                            t1=10.0)
+    inp = I.make_inputs(w, 300)
+    o = run_oracle(w, inp, max_steps=1024)
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=6)
     _compare(g, o, grad_tol=1e-13)

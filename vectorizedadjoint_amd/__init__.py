@@ -41,7 +41,9 @@ class VaProblem(C.Structure):
         ("t0", C.c_double), ("t1", C.c_double), ("dt0", C.c_double),
         ("abs_tol", C.c_double), ("rel_tol", C.c_double), ("system_constant", C.c_double),
         ("max_attempts", C.c_int32), ("max_steps", C.c_int32), ("mode", C.c_int32),
-        ("block_size", C.c_int32), ("lanes", C.c_int64),
+        ("block_size", C.c_int32), ("instances_per_wave", C.c_int32),
+        ("checkpoint_interval", C.c_int32), ("lanes", C.c_int64),
+        ("trajectory_budget_bytes", C.c_int64),
     ]
 
 
@@ -109,7 +111,9 @@ class Problem:
     max_steps: int = 4096
     mode: int = _K.MODE_PERSISTENT
     block_size: int = 0
+    instances_per_wave: int = 0
     lanes: int = 0
+    trajectory_budget_bytes: int = 0
 
     def to_c(self) -> VaProblem:
         pb = VaProblem()
@@ -123,6 +127,8 @@ class Problem:
         pb.system_constant = self.system_constant
         pb.max_attempts, pb.max_steps = self.max_attempts, self.max_steps
         pb.mode, pb.block_size, pb.lanes = self.mode, self.block_size, self.lanes
+        pb.instances_per_wave = self.instances_per_wave
+        pb.trajectory_budget_bytes = self.trajectory_budget_bytes
         return pb
 
     @classmethod

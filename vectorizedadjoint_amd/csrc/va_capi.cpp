@@ -83,7 +83,7 @@ struct va_handle {
     int device = 0;
     int cu_count = 0;
     int block = 256;
-    int64_t lanes = 0;
+    int64_t lanes_fwd = 0, lanes_rev = 0;  // instances in flight of the persistent kernels
     // persistent store
     double *traj = nullptr;
     // plain store + state
@@ -172,8 +172,8 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     a.status = h->status;
     a.counter = h->counters + 0;
     a.max_steps_seen = h->counters + 2;
-    const int64_t lanes = std::min<int64_t>(h->lanes, B);
-    VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->block), h->block, s));
+    const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
+    VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     return VA_OK;
 }
 
@@ -198,8 +198,8 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     a.mu = mu;
     a.nout = nout;
     a.counter = h->counters + 1;
-    const int64_t lanes = std::min<int64_t>(h->lanes, B);
-    VA_HIP(h->ks.rev(a, blocks_for(lanes, h->block), h->block, s));
+    const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
+    VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     return VA_OK;
 }
 
@@ -339,9 +339,16 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(VA_ERR_INVALID,
                     "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
     va::KernelSet ks;
-    if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive, &ks))
-        return fail(VA_ERR_UNSUPPORTED, "no kernels for system %d, N=%d, tableau %d, adaptive %d",
-                    pb->system, pb->n_state, pb->tableau, pb->adaptive);
+    if (pb->checkpoint_interval != 0)
+        return fail(VA_ERR_INVALID, "checkpoint_interval is reserved and must be 0");
+    if (pb->instances_per_wave < 0 || pb->instances_per_wave > 64)
+        return fail(VA_ERR_INVALID, "instances_per_wave must be in [0, 64]");
+    if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive,
+                         pb->instances_per_wave, &ks))
+        return fail(VA_ERR_UNSUPPORTED,
+                    "no kernels for system %d, N=%d, tableau %d, adaptive %d, "
+                    "instances_per_wave %d",
+                    pb->system, pb->n_state, pb->tableau, pb->adaptive, pb->instances_per_wave);
     if (pb->n_param != ks.np)
         return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
                     pb->system, pb->n_state, ks.np, pb->n_param);
@@ -363,7 +370,28 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(VA_ERR_INVALID, "block_size must be a multiple of 64 and <= 256");
     }
     const int64_t B = pb->batch;
-    h->lanes = pb->lanes > 0 ? pb->lanes : B;
+    if (pb->lanes > 0) {
+        h->lanes_fwd = h->lanes_rev = pb->lanes;
+    } else {
+        // every resident instance slot of the device
+        const int64_t per_block = ks.slots(h->block);
+        h->lanes_fwd = (int64_t)std::max(1, ks.occ_fwd(h->block)) * h->cu_count * per_block;
+        h->lanes_rev = (int64_t)std::max(1, ks.occ_rev(h->block)) * h->cu_count * per_block;
+    }
+    {
+        const size_t rows_b = (size_t)pb->max_steps + 1;
+        const size_t per_inst =
+            pb->mode == VA_MODE_PERSISTENT
+                ? rows_b * (ks.n + 2)
+                : rows_b * (2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n);
+        const double store = (double)sizeof(double) * (double)per_inst * (double)B;
+        if (pb->trajectory_budget_bytes > 0 && store > (double)pb->trajectory_budget_bytes) {
+            delete h;
+            return fail(VA_ERR_OUT_OF_MEMORY,
+                        "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
+                        store, (long long)pb->trajectory_budget_bytes);
+        }
+    }
 
     const size_t rows = (size_t)pb->max_steps + 1;
     hipError_t e = hipSuccess;

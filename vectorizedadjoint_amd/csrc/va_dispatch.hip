@@ -9,7 +9,8 @@ bool kernels_van_der_pol(int tableau, int adaptive, KernelSet *out);
 bool kernels_oscillator(int tableau, int adaptive, KernelSet *out);
 bool kernels_oscillator_fixed_k(int tableau, int adaptive, KernelSet *out);
 bool kernels_lotka_volterra_small(int n, int tableau, int adaptive, KernelSet *out);
-bool kernels_lotka_volterra_10(int tableau, int adaptive, KernelSet *out);
+bool kernels_lotka_volterra_10(int tableau, int adaptive, int instances_per_wave,
+                               KernelSet *out);
 
 __global__ void k_plain_finish(double *lambda_out, const double *lambda_in, double *mu,
                                const double *pbar, const int32_t *status, int64_t B, int n,
@@ -71,15 +72,18 @@ hipError_t launch_plain_cost(int cost, const double *cost_w, const double *yT, d
 
 enum { SYS_VDP = 1, SYS_OSC = 2, SYS_OSC_K = 3, SYS_GLV = 4 };
 
-bool get_kernels(int system, int n, int tableau, int adaptive, KernelSet *out)
+bool get_kernels(int system, int n, int tableau, int adaptive, int instances_per_wave,
+                 KernelSet *out)
 {
+    const bool one_lane = instances_per_wave == 0 || instances_per_wave == 64;
     switch (system) {
-    case SYS_VDP: return n == 2 && kernels_van_der_pol(tableau, adaptive, out);
-    case SYS_OSC: return n == 2 && kernels_oscillator(tableau, adaptive, out);
-    case SYS_OSC_K: return n == 2 && kernels_oscillator_fixed_k(tableau, adaptive, out);
+    case SYS_VDP: return n == 2 && one_lane && kernels_van_der_pol(tableau, adaptive, out);
+    case SYS_OSC: return n == 2 && one_lane && kernels_oscillator(tableau, adaptive, out);
+    case SYS_OSC_K:
+        return n == 2 && one_lane && kernels_oscillator_fixed_k(tableau, adaptive, out);
     case SYS_GLV:
-        if (n == 10) return kernels_lotka_volterra_10(tableau, adaptive, out);
-        return kernels_lotka_volterra_small(n, tableau, adaptive, out);
+        if (n == 10) return kernels_lotka_volterra_10(tableau, adaptive, instances_per_wave, out);
+        return one_lane && kernels_lotka_volterra_small(n, tableau, adaptive, out);
     default: return false;
     }
 }

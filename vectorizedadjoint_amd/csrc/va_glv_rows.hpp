@@ -1,0 +1,440 @@
+// va_glv_rows.hpp -- generalized Lotka-Volterra forward and reverse sweeps with one
+// instance per group of N lanes ("row per lane"), gfx950.
+//
+// Layout.  A wave64 holds G = 64/N instance groups (N = 10: 6 groups, 60 busy
+// lanes).  Lane i of a group owns state component i, row i of A (for the
+// growth rates g_i = r_i + sum_j A_ij x_j), column i of A (for the adjoint
+// term sum_k u_k A_ki) and row i of dJ/dA.  Stage vectors are exchanged
+// through a per-wave LDS slab: every lane writes its component, the group
+// reads the whole vector back as broadcast 16-byte loads.  Nothing crosses
+// waves, so there is no block barrier; LDS operations of one wave complete in
+// program order.
+//
+// Compared with one lane per instance this keeps A and dJ/dA in registers
+// (3N doubles per lane instead of 2N^2) and cuts the instruction count of one
+// instance-step by ~N, which is what bounds the long tail of instances that
+// need ~10x the median number of steps.
+//
+// Every sum is evaluated in the same order as in the one-lane kernels and the
+// CPU oracle (forward growth: j = 0..N-1 from r_i; adjoint: k = 0..N-1 from
+// w_i g_i; accumulations in stage order), so results are bit-identical.
+//
+// Reference path: lib/include/detail/runge_kutta.hpp:74-118 (forward loop),
+// lib/include/detail/backpropagation.hpp:83-158,231-254 (reverse step/loop),
+// examples/GeneralizedLotkaVolterra/main.cpp:105-119 (system, parameter layout).
+#pragma once
+
+#include "va_kernels_impl.hpp"
+
+namespace va {
+
+template <int NN>
+struct GlvRows {
+    static constexpr int G = NN;          // lanes per instance
+    static constexpr int GPW = 64 / NN;   // instance groups per wave
+    static constexpr int F = NN + 2;      // trajectory row (t, h, y[N])
+    static constexpr int WAVES = 4;       // block of 256 threads
+    static_assert(NN >= 2 && NN <= 32, "row-per-lane layout needs 2 <= N <= 32");
+};
+
+// Broadcast-gather of a group's vector from the wave's LDS slab.
+template <int NN>
+__device__ __forceinline__ void gather_vec(const double *slab, int gbase, double (&out)[NN])
+{
+    if constexpr (NN % 2 == 0) {
+        const double2 *s2 = reinterpret_cast<const double2 *>(slab + gbase);
+#pragma unroll
+        for (int j = 0; j < NN / 2; ++j) {
+            const double2 v = s2[j];
+            out[2 * j] = v.x;
+            out[2 * j + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NN; ++j) out[j] = slab[gbase + j];
+    }
+}
+
+template <int NN>
+__device__ __forceinline__ void exchange(double *slab, int lane, int gbase, double v,
+                                         double (&out)[NN])
+{
+    slab[lane] = v;
+    __builtin_amdgcn_wave_barrier();
+    gather_vec<NN>(slab, gbase, out);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NN>
+__device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r,
+                                             const double (&x)[NN])
+{
+    double g = r;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) g = fma(Arow[j], x[j], g);
+    return g;
+}
+
+// ---------------------------------------------------------------------------
+// Forward sweep (persistent, refill per instance group)
+// ---------------------------------------------------------------------------
+template <int NN, class Tab, bool Adaptive>
+__global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
+{
+    using L = GlvRows<NN>;
+    constexpr int F = L::F;
+    __shared__ double2 slab2[L::WAVES][32];
+    __shared__ int gidx[L::WAVES][L::GPW];
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane / NN, i = lane - grp * NN;
+    if (grp >= L::GPW) return;
+    double *slab = reinterpret_cast<double *>(slab2[wv]);
+    const int gbase = grp * NN;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+
+    double Arow[NN], r = 0.0, y = 0.0, K[Tab::SF], ynew = 0.0, xerr = 0.0;
+    double t = 0.0, dt = 0.0;
+    int64_t b = 0;
+    int32_t count = 0, rej = 0, tries = 0, st = LANE_OK;
+    bool new_step = true, have_k0 = false, need = true;
+    double *rows = nullptr;
+    double v[NN];
+
+    for (;;) {
+        if (need) {
+            if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
+            __builtin_amdgcn_wave_barrier();
+            b = gidx[wv][grp];
+            __builtin_amdgcn_wave_barrier();
+            if (b >= B) break;
+            r = a.p[(int64_t)i * B + b];
+#pragma unroll
+            for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+            y = a.y[(int64_t)i * B + b];
+            t = sp.t0;
+            dt = sp.dt0;
+            count = rej = tries = 0;
+            st = LANE_OK;
+            new_step = true;
+            have_k0 = false;
+            need = false;
+            rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+        }
+        bool finish = false;
+        if (new_step) {
+            const bool more = Adaptive ? less_with_sign(t, sp.t1, dt)
+                                       : less_eq_with_sign(t + sp.dt0, sp.t1, sp.dt0);
+            if (!more) {
+                finish = true;
+            } else if (count >= sp.cap) {
+                st = LANE_MAX_STEPS;
+                finish = true;
+            } else {
+                double *row = rows + (size_t)count * F;
+                if (i == 0) row[0] = t;
+                row[2 + i] = y;
+                if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
+                if (!Tab::FSAL || !have_k0) {
+                    exchange<NN>(slab, lane, gbase, y, v);
+                    K[0] = y * growth_row<NN>(Arow, r, v);
+                    have_k0 = true;
+                }
+                tries = 0;
+                new_step = false;
+            }
+        }
+        if (!finish) {
+            const double h = Adaptive ? dt : sp.dt0;
+#pragma unroll
+            for (int m = 1; m < Tab::SF; ++m) {
+                double X = y;
+#pragma unroll
+                for (int j = 0; j < Tab::SF; ++j)
+                    if (j < m && Tab::A[m][j] != 0.0) X = fma(Tab::A[m][j] * h, K[j], X);
+                exchange<NN>(slab, lane, gbase, X, v);
+                K[m] = X * growth_row<NN>(Arow, r, v);
+            }
+            ynew = y;
+#pragma unroll
+            for (int j = 0; j < Tab::S; ++j)
+                if (Tab::B[j] != 0.0) ynew = fma(Tab::B[j] * h, K[j], ynew);
+            if constexpr (Adaptive) {
+                xerr = 0.0;
+#pragma unroll
+                for (int j = 0; j < Tab::SF; ++j)
+                    if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
+                const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
+                exchange<NN>(slab, lane, gbase, fabs(xerr) / den, v);
+                double err = 0.0;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) err = (err < v[j]) ? v[j] : err;
+                if (err > 1.0) {
+                    dt = decrease_step<Tab>(dt, err);
+                    ++rej;
+                } else {
+                    if (i == 0) rows[(size_t)count * F + 1] = dt;
+                    t = t + dt;
+                    dt = increase_step<Tab>(dt, err);
+                    y = ynew;
+                    if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
+                    ++count;
+                    new_step = true;
+                }
+                ++tries;
+                if (tries > sp.max_attempts) {
+                    st = LANE_MAX_FAIL;
+                    finish = true;
+                }
+            } else {
+                if (i == 0) rows[(size_t)count * F + 1] = sp.dt0;
+                y = ynew;
+                if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
+                ++count;
+                t = sp.t0 + (double)count * sp.dt0;
+                new_step = true;
+            }
+        }
+        if (finish) {
+            if (st == LANE_OK) {
+                double *row = rows + (size_t)count * F;
+                if (i == 0) {
+                    row[0] = t;
+                    row[1] = 0.0;
+                }
+                row[2 + i] = y;
+                exchange<NN>(slab, lane, gbase, fabs(y), v);
+                bool fin = true;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) fin = fin && (v[j] <= 1.7976931348623157e308);
+                if (!fin) st = LANE_NONFINITE;
+            }
+            a.y[(int64_t)i * B + b] = y;
+            if (i == 0) {
+                a.n_acc[b] = count;
+                a.n_rej[b] = rej;
+                a.status[b] = st;
+                atomicMax(a.max_steps_seen, count);
+            }
+            need = true;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweep (persistent, refill per instance group)
+// ---------------------------------------------------------------------------
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
+{
+    using L = GlvRows<NN>;
+    constexpr int F = L::F, S = Tab::S, P = NN + NN * NN;
+    // per wave: S stage-input vectors + one scratch vector, 64 doubles each
+    __shared__ double2 xsl2[L::WAVES][S][32];
+    __shared__ double2 usl2[L::WAVES][32];
+    __shared__ int gidx[L::WAVES][L::GPW];
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane / NN, i = lane - grp * NN;
+    if (grp >= L::GPW) return;
+    const int gbase = grp * NN;
+    double *uslab = reinterpret_cast<double *>(usl2[wv]);
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const double qnan = __builtin_nan("");
+
+    double Arow[NN], Acol[NN], Abar[NN], r = 0.0, rbar = 0.0, lam = 0.0;
+    int64_t b = 0;
+    int32_t o = 0, n = -1;
+    bool need = true;
+    const double *rows = nullptr;
+    double v[NN];
+
+    for (;;) {
+        if (need) {
+            if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
+            __builtin_amdgcn_wave_barrier();
+            b = gidx[wv][grp];
+            __builtin_amdgcn_wave_barrier();
+            if (b >= B) break;
+            if (a.status[b] != LANE_OK) {
+                const int nseed = a.cost ? 1 : a.nout;
+                for (int oo = 0; oo < nseed; ++oo) {
+                    a.lambda[((int64_t)oo * NN + i) * B + b] = qnan;
+                    a.mu[((int64_t)oo * P + i) * B + b] = qnan;
+                    for (int j = 0; j < NN; ++j)
+                        a.mu[((int64_t)oo * P + NN * (i + 1) + j) * B + b] = qnan;
+                }
+                if (a.cost && i == 0) a.J[b] = qnan;
+                continue;
+            }
+            r = a.p[(int64_t)i * B + b];
+#pragma unroll
+            for (int j = 0; j < NN; ++j) {
+                Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+                Acol[j] = a.p[(int64_t)(NN * (j + 1) + i) * B + b];
+            }
+            rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+            o = 0;
+            if (a.cost) {
+                const double yT = a.yT[(int64_t)i * B + b];
+                // terminal cost: J = fold over i in order, lambda_i from y_i(T)
+                exchange<NN>(uslab, lane, gbase, yT, v);
+                double J = 0.0;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) {
+                    if (a.cost == COST_SUM) J = J + v[j];
+                    else if (a.cost == COST_SQUARED_NORM) J = fma(v[j], v[j], J);
+                    else J = fma(a.cost_w[j], v[j], J);
+                }
+                if (a.cost == COST_SUM) lam = 1.0;
+                else if (a.cost == COST_SQUARED_NORM) lam = 2.0 * yT;
+                else lam = a.cost_w[i];
+                if (i == 0) a.J[b] = J;
+            } else {
+                lam = a.lambda[(int64_t)i * B + b];
+            }
+            rbar = 0.0;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
+            n = a.n_acc[b] - 1;
+            need = false;
+        }
+        if (n >= 0) {
+            const double *row = rows + (size_t)n * F;
+            const double h = row[1];
+            const double yn = row[2 + i];
+            double X[S], g[S], K[S], Kb[S];
+            // recompute the stages from y_n (backpropagation.hpp:24-64)
+#pragma unroll
+            for (int m = 0; m < S; ++m) {
+                double x = yn;
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * h, K[j], x);
+                X[m] = x;
+                double *xs = reinterpret_cast<double *>(xsl2[wv][m]);
+                exchange<NN>(xs, lane, gbase, x, v);
+                g[m] = growth_row<NN>(Arow, r, v);
+                K[m] = x * g[m];
+            }
+            // seed (:104-112)
+#pragma unroll
+            for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h) * lam;
+            double ybar = lam;
+            // stages s..1 (:126-153)
+#pragma unroll
+            for (int m = S - 1; m >= 0; --m) {
+                const double w = Kb[m];
+                const double u = w * X[m];
+                exchange<NN>(uslab, lane, gbase, u, v);
+                double xi = w * g[m];
+#pragma unroll
+                for (int k = 0; k < NN; ++k) xi = fma(v[k], Acol[k], xi);
+                rbar = rbar + u;
+                gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
+#pragma unroll
+                for (int j = 0; j < NN; ++j) Abar[j] = Abar[j] + u * v[j];
+                ybar = ybar + xi;
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h, xi, Kb[j]);
+            }
+            lam = ybar;
+            --n;
+        }
+        if (n < 0) {
+            const int64_t lo = (int64_t)o * NN, po = (int64_t)o * P;
+            a.lambda[(lo + i) * B + b] = lam;
+            if (a.cost) {
+                a.mu[(po + i) * B + b] = 0.0 + rbar;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) a.mu[(po + NN * (i + 1) + j) * B + b] = 0.0 + Abar[j];
+            } else {
+                double *m0 = a.mu + (po + i) * B + b;
+                *m0 = *m0 + rbar;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) {
+                    double *mj = a.mu + (po + NN * (i + 1) + j) * B + b;
+                    *mj = *mj + Abar[j];
+                }
+            }
+            ++o;
+            if (a.cost || o >= a.nout) {
+                need = true;
+            } else {
+                lam = a.lambda[((int64_t)o * NN + i) * B + b];
+                rbar = 0.0;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
+                n = a.n_acc[b] - 1;
+            }
+        }
+    }
+}
+
+template <int NN, class Tab, bool Ad>
+struct LaunchGlvRows {
+    static hipError_t fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static hipError_t rev(const RevArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static int occ_fwd(int block)
+    {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_glv_rows_fwd<NN, Tab, Ad>, block,
+                                                         0) != hipSuccess)
+            return 1;
+        return nb;
+    }
+    static int occ_rev(int block)
+    {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_glv_rows_rev<NN, Tab>, block, 0) !=
+            hipSuccess)
+            return 1;
+        return nb;
+    }
+    // Instance slots of one block of `block` threads.
+    static int slots(int block) { return (block / 64) * GlvRows<NN>::GPW; }
+    static void apply(KernelSet *k)
+    {
+        k->fwd = &fwd;
+        k->rev = &rev;
+        k->occ_fwd = &occ_fwd;
+        k->occ_rev = &occ_rev;
+        k->slots = &slots;
+        k->layout = 1;
+    }
+};
+
+template <int NN>
+inline bool pick_glv_rows(int tableau, int adaptive, KernelSet *out)
+{
+    if (!pick_tableau<LotkaVolterra<NN>>(tableau, adaptive, out)) return false;
+    if (adaptive) {
+        switch (tableau) {
+        case TAB_CK54: LaunchGlvRows<NN, CashKarp54, true>::apply(out); return true;
+        case TAB_DOPRI5: LaunchGlvRows<NN, Dopri5, true>::apply(out); return true;
+        case TAB_F78: LaunchGlvRows<NN, Fehlberg78, true>::apply(out); return true;
+        default: return false;
+        }
+    }
+    switch (tableau) {
+    case TAB_EULER: LaunchGlvRows<NN, Euler, false>::apply(out); return true;
+    case TAB_RK4: LaunchGlvRows<NN, RK4, false>::apply(out); return true;
+    case TAB_CK54: LaunchGlvRows<NN, CashKarp54, false>::apply(out); return true;
+    case TAB_DOPRI5: LaunchGlvRows<NN, Dopri5, false>::apply(out); return true;
+    case TAB_F78: LaunchGlvRows<NN, Fehlberg78, false>::apply(out); return true;
+    default: return false;
+    }
+}
+
+}  // namespace va

@@ -82,11 +82,17 @@ struct KernelSet {
     hipError_t (*rev)(const RevArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_fwd)(const PlainFwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_rev)(const PlainRevArgs &, int grid, int block, hipStream_t);
-    int fwd_regs, rev_regs;  // informational
+    int (*occ_fwd)(int block);   // resident blocks per CU of the persistent kernels
+    int (*occ_rev)(int block);
+    int (*slots)(int block);     // instances one block holds at a time
+    int layout;                  // 0: one lane per instance, 1: one group of N lanes per instance
 };
 
 // Returns false when (system, n, tableau, adaptive) has no compiled kernels.
-bool get_kernels(int system, int n, int tableau, int adaptive, KernelSet *out);
+// instances_per_wave: 64 selects one lane per instance; 64/N (or 0 = default)
+// selects the row-per-lane layout where the system has one.
+bool get_kernels(int system, int n, int tableau, int adaptive, int instances_per_wave,
+                 KernelSet *out);
 
 hipError_t launch_plain_finish(double *lambda_out, const double *lambda_in, double *mu,
                                const double *pbar, const int32_t *status, int64_t B, int n, int np,

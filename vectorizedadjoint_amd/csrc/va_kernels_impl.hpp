@@ -487,6 +487,23 @@ struct Launch {
         hipLaunchKernelGGL((k_plain_rev<Sys, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
+    static int occ_fwd(int block)
+    {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_fwd<Sys, Tab, Ad>, block, 0) !=
+            hipSuccess)
+            return 1;
+        return nb;
+    }
+    static int occ_rev(int block)
+    {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_rev<Sys, Tab>, block, 0) !=
+            hipSuccess)
+            return 1;
+        return nb;
+    }
+    static int slots(int block) { return block; }
     static KernelSet make()
     {
         KernelSet k{};
@@ -500,6 +517,10 @@ struct Launch {
         k.rev = &rev;
         k.plain_fwd = &pfwd;
         k.plain_rev = &prev;
+        k.occ_fwd = &occ_fwd;
+        k.occ_rev = &occ_rev;
+        k.slots = &slots;
+        k.layout = 0;
         return k;
     }
 };

@@ -29,7 +29,7 @@ TABLEAU_COUNTS = {
     K.TAB_RK4: (4, 3, 3, 3, 4, 0),
     K.TAB_CASH_KARP54: (6, 5, 15, 15, 4, 5),
     K.TAB_DOPRI5: (6, 6, 20, 15, 5, 6),
-    K.TAB_FEHLBERG78: (13, 12, 51, 51, 7, 4),
+    K.TAB_FEHLBERG78: (13, 12, 55, 55, 7, 4),
 }
 
 TABLEAU_NAMES = {
