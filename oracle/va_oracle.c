@@ -43,18 +43,18 @@ static double orc_log_pos(double x)
     }
     double s = (m - 1.0) / (m + 1.0);
     double z = s * s;
-    /* 1 + z/3 + z^2/5 + ... + z^11/23 */
-    double q = 1.0 / 23.0;
-    q = fma(q, z, 1.0 / 21.0);
-    q = fma(q, z, 1.0 / 19.0);
-    q = fma(q, z, 1.0 / 17.0);
-    q = fma(q, z, 1.0 / 15.0);
-    q = fma(q, z, 1.0 / 13.0);
-    q = fma(q, z, 1.0 / 11.0);
-    q = fma(q, z, 1.0 / 9.0);
-    q = fma(q, z, 1.0 / 7.0);
-    q = fma(q, z, 1.0 / 5.0);
-    q = fma(q, z, 1.0 / 3.0);
+    /* q = 1/3 + z/5 + ... + z^10/23 by Estrin's scheme (pairs, then z^2, z^4, z^8) */
+    double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+    double p01 = fma(1.0 / 5.0, z, 1.0 / 3.0);
+    double p23 = fma(1.0 / 9.0, z, 1.0 / 7.0);
+    double p45 = fma(1.0 / 13.0, z, 1.0 / 11.0);
+    double p67 = fma(1.0 / 17.0, z, 1.0 / 15.0);
+    double p89 = fma(1.0 / 21.0, z, 1.0 / 19.0);
+    double p03 = fma(p23, z2, p01);
+    double p47 = fma(p67, z2, p45);
+    double p8a = fma(1.0 / 23.0, z2, p89);
+    double p07 = fma(p47, z4, p03);
+    double q = fma(p8a, z8, p07);
     double zq = z * q;
     double s2 = s + s;
     double lm = fma(s2, zq, s2); /* 2s (1 + z q) */
@@ -70,20 +70,21 @@ static double orc_exp(double t)
     double k = rint(t * ORC_INV_LN2);
     double r = fma(-k, ORC_LN2_HI, t);
     r = fma(-k, ORC_LN2_LO, r);
-    double q = 1.0 / 6227020800.0; /* 1/13! */
-    q = fma(q, r, 1.0 / 479001600.0);
-    q = fma(q, r, 1.0 / 39916800.0);
-    q = fma(q, r, 1.0 / 3628800.0);
-    q = fma(q, r, 1.0 / 362880.0);
-    q = fma(q, r, 1.0 / 40320.0);
-    q = fma(q, r, 1.0 / 5040.0);
-    q = fma(q, r, 1.0 / 720.0);
-    q = fma(q, r, 1.0 / 120.0);
-    q = fma(q, r, 1.0 / 24.0);
-    q = fma(q, r, 1.0 / 6.0);
-    q = fma(q, r, 0.5);
-    q = fma(q, r, 1.0);
-    q = fma(q, r, 1.0);
+    /* sum_{i=0}^{13} r^i / i! by Estrin's scheme (pairs, then r^2, r^4, r^8) */
+    double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    double e01 = fma(1.0, r, 1.0);
+    double e23 = fma(1.0 / 6.0, r, 0.5);
+    double e45 = fma(1.0 / 120.0, r, 1.0 / 24.0);
+    double e67 = fma(1.0 / 5040.0, r, 1.0 / 720.0);
+    double e89 = fma(1.0 / 362880.0, r, 1.0 / 40320.0);
+    double eab = fma(1.0 / 39916800.0, r, 1.0 / 3628800.0);
+    double ecd = fma(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
+    double e03 = fma(e23, r2, e01);
+    double e47 = fma(e67, r2, e45);
+    double e8b = fma(eab, r2, e89);
+    double e07 = fma(e47, r4, e03);
+    double e8d = fma(ecd, r4, e8b);
+    double q = fma(e8d, r8, e07);
     return ldexp(q, (int)k);
 }
 
@@ -223,6 +224,18 @@ int orc_get_tableau(int id, orc_tableau *t)
 /* their hand-derived vector-Jacobian products (these replace the AADC tape, */
 /* AadData.hpp:124-171,291-330).                                             */
 /* ------------------------------------------------------------------------ */
+
+/* Lotka-Volterra growth rate g_i = r_i + sum_j A_ij x_j, summed as two
+ * interleaved partial sums (even j starting from r_i, odd j starting from 0)
+ * added at the end; the HIP kernels use the same order. */
+static double glv_growth(int n, const double *Ai, double r, const double *x)
+{
+    double ge = r, go = 0.0;
+    for (int j = 0; j < n; j += 2) ge = fma(Ai[j], x[j], ge);
+    for (int j = 1; j < n; j += 2) go = fma(Ai[j], x[j], go);
+    return ge + go;
+}
+
 void orc_rhs(int system, int n, const double *x, const double *p, double t, double K,
              double *f)
 {
@@ -244,12 +257,7 @@ void orc_rhs(int system, int n, const double *x, const double *p, double t, doub
         break;
     case ORC_SYS_LOTKA_VOLTERRA:
     case ORC_SYS_LOTKA_VOLTERRA_SHARED: /* examples/GeneralizedLotkaVolterra/main.cpp:109-118 */
-        for (int i = 0; i < n; ++i) {
-            const double *Ai = p + (size_t)n * (i + 1);
-            double g = p[i];
-            for (int j = 0; j < n; ++j) g = fma(Ai[j], x[j], g);
-            f[i] = x[i] * g;
-        }
+        for (int i = 0; i < n; ++i) f[i] = x[i] * glv_growth(n, p + (size_t)n * (i + 1), p[i], x);
         break;
     default:
         for (int i = 0; i < n; ++i) f[i] = NAN;
@@ -289,20 +297,22 @@ void orc_vjp(int system, int n, int np, const double *x, const double *p, double
     case ORC_SYS_LOTKA_VOLTERRA:
     case ORC_SYS_LOTKA_VOLTERRA_SHARED: {
         /* f_i = x_i g_i, g_i = r_i + sum_j A_ij x_j
-         * wJx_j = w_j g_j + sum_i (w_i x_i) A_ij;  r_bar_i = w_i x_i;  A_bar_ij = w_i x_i x_j */
+         * wJx_j = w_j g_j + sum_i (w_i x_i) A_ij;  r_bar_i = w_i x_i;  A_bar_ij = w_i x_i x_j
+         * The sum over i is split like the growth sum: even i starting from
+         * w_j g_j, odd i starting from 0, added at the end. */
         double u[ORC_MAX_N];
         for (int i = 0; i < n; ++i) u[i] = w[i] * x[i];
         for (int j = 0; j < n; ++j) {
-            const double *Aj = p + (size_t)n * (j + 1);
-            double g = p[j];
-            for (int k = 0; k < n; ++k) g = fma(Aj[k], x[k], g);
-            double acc = w[j] * g;
+            double g = glv_growth(n, p + (size_t)n * (j + 1), p[j], x);
+            double ae = w[j] * g, ao = 0.0;
 #if ORC_MUTANT == 1 /* planted transposed operand, test builds only */
-            for (int i = 0; i < n; ++i) acc = fma(u[i], p[(size_t)n * (j + 1) + i], acc);
+            for (int i = 0; i < n; i += 2) ae = fma(u[i], p[(size_t)n * (j + 1) + i], ae);
+            for (int i = 1; i < n; i += 2) ao = fma(u[i], p[(size_t)n * (j + 1) + i], ao);
 #else
-            for (int i = 0; i < n; ++i) acc = fma(u[i], p[(size_t)n * (i + 1) + j], acc);
+            for (int i = 0; i < n; i += 2) ae = fma(u[i], p[(size_t)n * (i + 1) + j], ae);
+            for (int i = 1; i < n; i += 2) ao = fma(u[i], p[(size_t)n * (i + 1) + j], ao);
 #endif
-            wJx[j] = acc;
+            wJx[j] = ae + ao;
         }
         for (int i = 0; i < n; ++i) {
             wJp[i] = u[i];

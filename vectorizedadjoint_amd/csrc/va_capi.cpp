@@ -233,7 +233,7 @@ va_status forward_plain(va_handle *h, double *y, const double *p, hipStream_t s)
     a.n_rej = h->n_rej;
     a.status = h->status;
     a.st = plain_state(h);
-    const int grid = blocks_for(B, h->block);
+    const int grid = blocks_for(B, h->ks.slots(h->block));
     const int32_t last = h->pb.max_steps + 1;
     const int chunk = 8;
     for (int32_t step = 0; step <= last; step += chunk) {
@@ -273,13 +273,13 @@ va_status reverse_plain_seed(va_handle *h, const double *p, double *lambda, doub
     a.status = h->status;
     a.lambda = lambda;
     a.st = plain_state(h);
-    const int grid = blocks_for(B, h->block);
+    const int grid = blocks_for(B, h->ks.slots(h->block));
     for (int32_t k = 0; k < smax; ++k) {
         a.k = k;
         VA_HIP(h->ks.plain_rev(a, grid, h->block, s));
     }
     VA_HIP(va::launch_plain_finish(lambda, lambda, mu, h->pl_pbar, h->status, B, h->ks.n,
-                                   h->ks.np, grid, h->block, s));
+                                   h->ks.np, blocks_for(B, 256), 256, s));
     return VA_OK;
 }
 

@@ -140,17 +140,19 @@ VA_HD double log_pos(double x)
     }
     double s = (m - 1.0) / (m + 1.0);
     double z = s * s;
-    double q = 1.0 / 23.0;
-    q = fma(q, z, 1.0 / 21.0);
-    q = fma(q, z, 1.0 / 19.0);
-    q = fma(q, z, 1.0 / 17.0);
-    q = fma(q, z, 1.0 / 15.0);
-    q = fma(q, z, 1.0 / 13.0);
-    q = fma(q, z, 1.0 / 11.0);
-    q = fma(q, z, 1.0 / 9.0);
-    q = fma(q, z, 1.0 / 7.0);
-    q = fma(q, z, 1.0 / 5.0);
-    q = fma(q, z, 1.0 / 3.0);
+    // q = 1/3 + z/5 + ... + z^10/23 by Estrin's scheme (short dependency chains:
+    // a dependent FP64 FMA costs ~32 cycles on gfx950)
+    double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+    double p01 = fma(1.0 / 5.0, z, 1.0 / 3.0);
+    double p23 = fma(1.0 / 9.0, z, 1.0 / 7.0);
+    double p45 = fma(1.0 / 13.0, z, 1.0 / 11.0);
+    double p67 = fma(1.0 / 17.0, z, 1.0 / 15.0);
+    double p89 = fma(1.0 / 21.0, z, 1.0 / 19.0);
+    double p03 = fma(p23, z2, p01);
+    double p47 = fma(p67, z2, p45);
+    double p8a = fma(1.0 / 23.0, z2, p89);
+    double p07 = fma(p47, z4, p03);
+    double q = fma(p8a, z8, p07);
     double zq = z * q;
     double s2 = s + s;
     double lm = fma(s2, zq, s2);
@@ -166,20 +168,21 @@ VA_HD double exp_(double t)
     double k = rint(t * VA_INV_LN2);
     double r = fma(-k, VA_LN2_HI, t);
     r = fma(-k, VA_LN2_LO, r);
-    double q = 1.0 / 6227020800.0;
-    q = fma(q, r, 1.0 / 479001600.0);
-    q = fma(q, r, 1.0 / 39916800.0);
-    q = fma(q, r, 1.0 / 3628800.0);
-    q = fma(q, r, 1.0 / 362880.0);
-    q = fma(q, r, 1.0 / 40320.0);
-    q = fma(q, r, 1.0 / 5040.0);
-    q = fma(q, r, 1.0 / 720.0);
-    q = fma(q, r, 1.0 / 120.0);
-    q = fma(q, r, 1.0 / 24.0);
-    q = fma(q, r, 1.0 / 6.0);
-    q = fma(q, r, 0.5);
-    q = fma(q, r, 1.0);
-    q = fma(q, r, 1.0);
+    // sum_{i=0}^{13} r^i / i! by Estrin's scheme
+    double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    double e01 = fma(1.0, r, 1.0);
+    double e23 = fma(1.0 / 6.0, r, 0.5);
+    double e45 = fma(1.0 / 120.0, r, 1.0 / 24.0);
+    double e67 = fma(1.0 / 5040.0, r, 1.0 / 720.0);
+    double e89 = fma(1.0 / 362880.0, r, 1.0 / 40320.0);
+    double eab = fma(1.0 / 39916800.0, r, 1.0 / 3628800.0);
+    double ecd = fma(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
+    double e03 = fma(e23, r2, e01);
+    double e47 = fma(e67, r2, e45);
+    double e8b = fma(eab, r2, e89);
+    double e07 = fma(e47, r4, e03);
+    double e8d = fma(ecd, r4, e8b);
+    double q = fma(e8d, r8, e07);
     return ldexp(q, (int)k);
 }
 
@@ -236,6 +239,28 @@ VA_HD double increase_step(double h, double err)
         h = h * (0.9 * ctrl_pow(err, -1.0 / (double)Tab::ORDER));
     }
     return h;
+}
+
+// Accept/reject decision and the next step size with ONE pow evaluation:
+// decrease_step and increase_step above, with the pow argument and exponent
+// selected first.  Lanes of a wave that take different decisions then share a
+// single pow instead of executing one per branch; every value is computed with
+// the same operations as in the two functions (bit-identical).
+template <class Tab>
+VA_HD bool control_step(double h, double err, double &h_next)
+{
+    constexpr double lo = FivePow<Tab::ORDER>::inv();
+    constexpr double e_dec = -1.0 / (double)(Tab::ERR_ORDER - 1);
+    constexpr double e_inc = -1.0 / (double)Tab::ORDER;
+    const bool accept = !(err > 1.0);
+    const double base = accept ? ((lo < err) ? err : lo) : err;
+    const double f = 0.9 * ctrl_pow(base, accept ? e_inc : e_dec);
+    if (!accept) {
+        h_next = h * ((f < 0.2) ? 0.2 : f);
+    } else {
+        h_next = (err < 0.5) ? h * f : h;
+    }
+    return accept;
 }
 
 // ---------------------------------------------------------------------------
@@ -342,12 +367,16 @@ struct LotkaVolterra {
 #pragma unroll
             for (int j = 0; j < NN; ++j) q.A[i][j] = p[(int64_t)(NN * (i + 1) + j) * B + b];
     }
+    // g_i = r_i + sum_j A_ij x_j as two interleaved partial sums (even j from
+    // r_i, odd j from 0), added at the end.
     VA_HD static double growth(const Params &q, const double *x, int i)
     {
-        double g = q.r[i];
+        double ge = q.r[i], go = 0.0;
 #pragma unroll
-        for (int j = 0; j < NN; ++j) g = fma(q.A[i][j], x[j], g);
-        return g;
+        for (int j = 0; j < NN; j += 2) ge = fma(q.A[i][j], x[j], ge);
+#pragma unroll
+        for (int j = 1; j < NN; j += 2) go = fma(q.A[i][j], x[j], go);
+        return ge + go;
     }
     VA_HD static void rhs(const Params &q, const double *x, double *f)
     {
@@ -363,10 +392,12 @@ struct LotkaVolterra {
         for (int i = 0; i < NN; ++i) u[i] = w[i] * x[i];
 #pragma unroll
         for (int j = 0; j < NN; ++j) {
-            double acc = w[j] * growth(q, x, j);
+            double ae = w[j] * growth(q, x, j), ao = 0.0;
 #pragma unroll
-            for (int i = 0; i < NN; ++i) acc = fma(u[i], q.A[i][j], acc);
-            wjx[j] = acc;
+            for (int i = 0; i < NN; i += 2) ae = fma(u[i], q.A[i][j], ae);
+#pragma unroll
+            for (int i = 1; i < NN; i += 2) ao = fma(u[i], q.A[i][j], ao);
+            wjx[j] = ae + ao;
         }
 #pragma unroll
         for (int i = 0; i < NN; ++i) {

@@ -55,6 +55,36 @@ __device__ __forceinline__ void gather_vec(const double *slab, int gbase, double
     }
 }
 
+// Maximum of a group's error ratios.  Every value is >= +0 or NaN; fmax is
+// exact, ignores NaN operands like the sequential (m < e) ? e : m fold, and is
+// order-independent on such values, so a balanced tree gives the same result
+// with a dependency depth of log2(N).
+template <int NN>
+__device__ __forceinline__ double group_max(const double (&v)[NN])
+{
+    double t[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) t[j] = v[j];
+#pragma unroll
+    for (int w = 1; w < NN; w *= 2)
+#pragma unroll
+        for (int j = 0; j + w < NN; j += 2 * w) t[j] = fmax(t[j], t[j + w]);
+    return fmax(t[0], 0.0);
+}
+
+// Long-running instances set the job's critical path: a wave holding one gets
+// SIMD issue priority so it runs near its lone-wave latency while other waves
+// fill the remaining issue slots.
+__device__ __forceinline__ void prioritise_long(bool long_running)
+{
+    if (__builtin_amdgcn_ballot_w64(long_running) != 0)
+        __builtin_amdgcn_s_setprio(2);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
+constexpr int VA_LONG_ATTEMPTS = 160;  // ~2x the median attempts of an instance
+
 template <int NN>
 __device__ __forceinline__ void exchange(double *slab, int lane, int gbase, double v,
                                          double (&out)[NN])
@@ -65,14 +95,32 @@ __device__ __forceinline__ void exchange(double *slab, int lane, int gbase, doub
     __builtin_amdgcn_wave_barrier();
 }
 
+// g_i = r_i + sum_j A_ij x_j: two interleaved partial sums (even j from r_i,
+// odd j from 0) added at the end -- half the dependent-FMA depth, same order as
+// the one-lane kernels and the oracle.
 template <int NN>
 __device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r,
                                              const double (&x)[NN])
 {
-    double g = r;
+    double ge = r, go = 0.0;
 #pragma unroll
-    for (int j = 0; j < NN; ++j) g = fma(Arow[j], x[j], g);
-    return g;
+    for (int j = 0; j < NN; j += 2) ge = fma(Arow[j], x[j], ge);
+#pragma unroll
+    for (int j = 1; j < NN; j += 2) go = fma(Arow[j], x[j], go);
+    return ge + go;
+}
+
+// xi_j = w_j g_j + sum_k u_k A_kj, split the same way.
+template <int NN>
+__device__ __forceinline__ double adjoint_col(const double (&Acol)[NN], double wg,
+                                              const double (&u)[NN])
+{
+    double ae = wg, ao = 0.0;
+#pragma unroll
+    for (int k = 0; k < NN; k += 2) ae = fma(u[k], Acol[k], ae);
+#pragma unroll
+    for (int k = 1; k < NN; k += 2) ao = fma(u[k], Acol[k], ao);
+    return ae + ao;
 }
 
 // ---------------------------------------------------------------------------
@@ -103,6 +151,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
     double v[NN];
 
     for (;;) {
+        prioritise_long(!need && count + rej > VA_LONG_ATTEMPTS);
         if (need) {
             if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
             __builtin_amdgcn_wave_barrier();
@@ -167,21 +216,19 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
                     if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
                 const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
                 exchange<NN>(slab, lane, gbase, fabs(xerr) / den, v);
-                double err = 0.0;
-#pragma unroll
-                for (int j = 0; j < NN; ++j) err = (err < v[j]) ? v[j] : err;
-                if (err > 1.0) {
-                    dt = decrease_step<Tab>(dt, err);
+                const double err = group_max<NN>(v);
+                double dt_next;
+                if (!control_step<Tab>(dt, err, dt_next)) {
                     ++rej;
                 } else {
                     if (i == 0) rows[(size_t)count * F + 1] = dt;
                     t = t + dt;
-                    dt = increase_step<Tab>(dt, err);
                     y = ynew;
                     if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
                     ++count;
                     new_step = true;
                 }
+                dt = dt_next;
                 ++tries;
                 if (tries > sp.max_attempts) {
                     st = LANE_MAX_FAIL;
@@ -252,6 +299,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
     double v[NN];
 
     for (;;) {
+        prioritise_long(!need && n > VA_LONG_ATTEMPTS);
         if (need) {
             if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
             __builtin_amdgcn_wave_barrier();
@@ -329,9 +377,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
                 const double w = Kb[m];
                 const double u = w * X[m];
                 exchange<NN>(uslab, lane, gbase, u, v);
-                double xi = w * g[m];
-#pragma unroll
-                for (int k = 0; k < NN; ++k) xi = fma(v[k], Acol[k], xi);
+                const double xi = adjoint_col<NN>(Acol, w * g[m], v);
                 rbar = rbar + u;
                 gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
 #pragma unroll
@@ -374,6 +420,211 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Plain form in the same layout: one launch per accepted step, no refill, full
+// (t, h, y, K_0..K_{S-1}) store in [step][field][instance] order.
+// ---------------------------------------------------------------------------
+template <int NN, class Tab, bool Adaptive>
+__global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
+{
+    using L = GlvRows<NN>;
+    constexpr int S = Tab::S;
+    constexpr int F = 2 + NN + S * NN;
+    __shared__ double2 slab2[L::WAVES][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane / NN, i = lane - grp * NN;
+    if (grp >= L::GPW) return;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int64_t b = ((int64_t)blockIdx.x * L::WAVES + wv) * L::GPW + grp;
+    if (b >= B || a.st.done[b]) return;
+    double *slab = reinterpret_cast<double *>(slab2[wv]);
+    const int gbase = grp * NN;
+    const int32_t step = a.step;
+    double v[NN], Arow[NN], K[Tab::SF];
+    const double r = a.p[(int64_t)i * B + b];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+    double y = a.y[(int64_t)i * B + b];
+    double t = step == 0 ? sp.t0 : a.st.t[b];
+    double dt = step == 0 ? sp.dt0 : a.st.dt[b];
+    int32_t rej = step == 0 ? 0 : a.n_rej[b];
+    double *row = a.traj + (size_t)step * F * B + b;
+    const bool more = Adaptive ? less_with_sign(t, sp.t1, dt)
+                               : less_eq_with_sign(t + sp.dt0, sp.t1, sp.dt0);
+    if (!more || step >= sp.cap) {
+        int32_t st = LANE_OK;
+        if (more) {
+            st = LANE_MAX_STEPS;
+        } else {
+            if (i == 0) {
+                row[0] = t;
+                row[(size_t)B] = 0.0;
+            }
+            row[(size_t)(2 + i) * B] = y;
+            exchange<NN>(slab, lane, gbase, fabs(y), v);
+            bool fin = true;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) fin = fin && (v[j] <= 1.7976931348623157e308);
+            if (!fin) st = LANE_NONFINITE;
+        }
+        if (i == 0) {
+            a.status[b] = st;
+            a.n_acc[b] = step;
+            a.n_rej[b] = rej;
+            a.st.done[b] = 1;
+            atomicSub(a.st.active, 1);
+            atomicMax(a.st.max_steps_seen, step);
+        }
+        return;
+    }
+    if (i == 0) row[0] = t;
+    row[(size_t)(2 + i) * B] = y;
+    if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
+    if (Tab::FSAL && step > 0) {
+        K[0] = a.st.k0[(int64_t)i * B + b];
+    } else {
+        exchange<NN>(slab, lane, gbase, y, v);
+        K[0] = y * growth_row<NN>(Arow, r, v);
+    }
+    double ynew = y;
+    int32_t tries = 0;
+    for (;;) {
+        const double h = Adaptive ? dt : sp.dt0;
+#pragma unroll
+        for (int m = 1; m < Tab::SF; ++m) {
+            double X = y;
+#pragma unroll
+            for (int j = 0; j < Tab::SF; ++j)
+                if (j < m && Tab::A[m][j] != 0.0) X = fma(Tab::A[m][j] * h, K[j], X);
+            exchange<NN>(slab, lane, gbase, X, v);
+            K[m] = X * growth_row<NN>(Arow, r, v);
+        }
+        ynew = y;
+#pragma unroll
+        for (int j = 0; j < Tab::S; ++j)
+            if (Tab::B[j] != 0.0) ynew = fma(Tab::B[j] * h, K[j], ynew);
+        if constexpr (!Adaptive) {
+            if (i == 0) row[(size_t)B] = sp.dt0;
+            t = sp.t0 + (double)(step + 1) * sp.dt0;
+            break;
+        } else {
+            double xerr = 0.0;
+#pragma unroll
+            for (int j = 0; j < Tab::SF; ++j)
+                if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
+            const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
+            exchange<NN>(slab, lane, gbase, fabs(xerr) / den, v);
+            const double err = group_max<NN>(v);
+            double dt_next;
+            const bool ok = control_step<Tab>(dt, err, dt_next);
+            if (!ok) {
+                ++rej;
+            } else {
+                if (i == 0) row[(size_t)B] = dt;
+                t = t + dt;
+            }
+            dt = dt_next;
+            ++tries;
+            if (tries > sp.max_attempts) {
+                if (ok) a.y[(int64_t)i * B + b] = ynew;
+                if (i == 0) {
+                    a.status[b] = LANE_MAX_FAIL;
+                    a.n_acc[b] = step + (ok ? 1 : 0);
+                    a.n_rej[b] = rej;
+                    a.st.done[b] = 1;
+                    atomicSub(a.st.active, 1);
+                }
+                return;
+            }
+            if (ok) break;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < S; ++m) row[(size_t)(2 + NN + m * NN + i) * B] = K[m];
+    a.y[(int64_t)i * B + b] = ynew;
+    if constexpr (Tab::FSAL) a.st.k0[(int64_t)i * B + b] = K[Tab::SF - 1];
+    if (i == 0) {
+        a.st.t[b] = t;
+        a.st.dt[b] = dt;
+        a.n_rej[b] = rej;
+        a.n_acc[b] = step + 1;
+        a.status[b] = LANE_OK;
+    }
+}
+
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
+{
+    using L = GlvRows<NN>;
+    constexpr int S = Tab::S;
+    constexpr int F = 2 + NN + S * NN;
+    __shared__ double2 xsl2[L::WAVES][S][32];
+    __shared__ double2 usl2[L::WAVES][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane / NN, i = lane - grp * NN;
+    if (grp >= L::GPW) return;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int64_t b = ((int64_t)blockIdx.x * L::WAVES + wv) * L::GPW + grp;
+    if (b >= B || a.status[b] != LANE_OK) return;
+    const int32_t n = a.n_acc[b] - 1 - a.k;
+    if (n < 0) return;
+    const int gbase = grp * NN;
+    double *uslab = reinterpret_cast<double *>(usl2[wv]);
+    double v[NN], Arow[NN], Acol[NN];
+    const double r = a.p[(int64_t)i * B + b];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+        Acol[j] = a.p[(int64_t)(NN * (j + 1) + i) * B + b];
+    }
+    const double *row = a.traj + (size_t)n * F * B + b;
+    const double h = row[(size_t)B];
+    const double yn = row[(size_t)(2 + i) * B];
+    double X[S], g[S], K[S], Kb[S];
+#pragma unroll
+    for (int m = 0; m < S; ++m) K[m] = row[(size_t)(2 + NN + m * NN + i) * B];
+#pragma unroll
+    for (int m = 0; m < S; ++m) {
+        double x = yn;
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * h, K[j], x);
+        X[m] = x;
+        double *xs = reinterpret_cast<double *>(xsl2[wv][m]);
+        exchange<NN>(xs, lane, gbase, x, v);
+        g[m] = growth_row<NN>(Arow, r, v);
+    }
+    double lam = a.lambda[(int64_t)i * B + b];
+    double rbar = a.st.pbar[(int64_t)i * B + b];
+    double Abar[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) Abar[j] = a.st.pbar[(int64_t)(NN * (i + 1) + j) * B + b];
+#pragma unroll
+    for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h) * lam;
+    double ybar = lam;
+#pragma unroll
+    for (int m = S - 1; m >= 0; --m) {
+        const double w = Kb[m];
+        const double u = w * X[m];
+        exchange<NN>(uslab, lane, gbase, u, v);
+        const double xi = adjoint_col<NN>(Acol, w * g[m], v);
+        rbar = rbar + u;
+        gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
+#pragma unroll
+        for (int j = 0; j < NN; ++j) Abar[j] = Abar[j] + u * v[j];
+        ybar = ybar + xi;
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h, xi, Kb[j]);
+    }
+    a.lambda[(int64_t)i * B + b] = ybar;
+    a.st.pbar[(int64_t)i * B + b] = rbar;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) a.st.pbar[(int64_t)(NN * (i + 1) + j) * B + b] = Abar[j];
+}
+
 template <int NN, class Tab, bool Ad>
 struct LaunchGlvRows {
     static hipError_t fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
@@ -384,6 +635,17 @@ struct LaunchGlvRows {
     static hipError_t rev(const RevArgs &a, int grid, int block, hipStream_t s)
     {
         hipLaunchKernelGGL((k_glv_rows_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static hipError_t pfwd(const PlainFwdArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_plain_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s,
+                           a);
+        return hipGetLastError();
+    }
+    static hipError_t prev(const PlainRevArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_plain_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
     static int occ_fwd(int block)
@@ -408,6 +670,8 @@ struct LaunchGlvRows {
     {
         k->fwd = &fwd;
         k->rev = &rev;
+        k->plain_fwd = &pfwd;
+        k->plain_rev = &prev;
         k->occ_fwd = &occ_fwd;
         k->occ_rev = &occ_rev;
         k->slots = &slots;

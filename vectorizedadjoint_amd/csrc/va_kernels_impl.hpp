@@ -96,13 +96,13 @@ __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
             R::attempt(q, y, K, h, ynew, xerr);
             if constexpr (Adaptive) {
                 const double err = R::error_norm(sp.atol, sp.rtol, h, y, K[0], xerr);
-                if (err > 1.0) {
-                    dt = decrease_step<Tab>(dt, err);
+                double dt_next;
+                const bool accept = control_step<Tab>(dt, err, dt_next);
+                if (!accept) {
                     ++rej;
                 } else {
                     rows[(size_t)count * F + 1] = dt;
                     t = t + dt;
-                    dt = increase_step<Tab>(dt, err);
 #pragma unroll
                     for (int i = 0; i < N; ++i) y[i] = ynew[i];
                     if constexpr (Tab::FSAL) {
@@ -112,6 +112,7 @@ __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
                     ++count;
                     new_step = true;
                 }
+                dt = dt_next;
                 ++tries;
                 if (tries > sp.max_attempts) {
                     st = LANE_MAX_FAIL;
@@ -355,16 +356,15 @@ __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
         for (;;) {
             R::attempt(q, y, K, dt, ynew, xerr);
             const double err = R::error_norm(sp.atol, sp.rtol, dt, y, K[0], xerr);
-            bool ok = false;
-            if (err > 1.0) {
-                dt = decrease_step<Tab>(dt, err);
+            double dt_next;
+            const bool ok = control_step<Tab>(dt, err, dt_next);
+            if (!ok) {
                 ++rej;
             } else {
                 row[(size_t)B] = dt;
                 t = t + dt;
-                dt = increase_step<Tab>(dt, err);
-                ok = true;
             }
+            dt = dt_next;
             ++tries;
             if (tries > sp.max_attempts) {
                 if (ok) {
