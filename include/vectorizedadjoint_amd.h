@@ -154,6 +154,15 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
                    double *dJdp, int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
                    void *stream);
 
+/* Driver::GetT/GetTime/GetDt/GetState (lib/include/Driver.hpp:53-65,
+ * StateStorage.hpp:17-42): copies the rows stored for one instance by the last
+ * va_forward into HOST memory, row n = (t_n, h_n, y_n[0..N-1]) with h_n the
+ * accepted step taken from t_n (0 on the final row).  At most max_rows rows
+ * are copied; *n_rows receives the number stored (accepted steps + 1).
+ * Synchronises the stream. */
+va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
+                        int64_t *n_rows, void *stream);
+
 /* Largest accepted-step count of the last forward pass (synchronises the stream). */
 va_status va_max_steps_seen(va_handle *h, int32_t *out, void *stream);
 

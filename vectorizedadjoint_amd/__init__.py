@@ -29,7 +29,7 @@ ABI_VERSION = 1
 
 C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
-    "va_sensitivity_matrix", "va_solve", "va_max_steps_seen", "va_last_error",
+    "va_sensitivity_matrix", "va_solve", "va_trajectory", "va_max_steps_seen", "va_last_error",
 )
 
 
@@ -79,10 +79,11 @@ def lib() -> C.CDLL:
         L.va_sensitivity_matrix.argtypes = [vp, vp, vp, vp]
         L.va_solve.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.va_max_steps_seen.argtypes = [vp, C.POINTER(C.c_int32), vp]
+        L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
         L.va_last_error.argtypes = []
         L.va_last_error.restype = C.c_char_p
         for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
-                  "va_max_steps_seen"):
+                  "va_trajectory", "va_max_steps_seen"):
             getattr(L, f).restype = C.c_int
         _lib_handle = L
     return _lib_handle
@@ -245,6 +246,17 @@ class Solver:
                               _ptr(out["dJdp"]), _ptr(out.get("n_acc")), _ptr(out.get("n_rej")),
                               _ptr(out.get("status")), _stream(stream)))
         return out
+
+    def trajectory(self, instance: int, stream=None):
+        """Rows (t_n, h_n, y_n) stored for one instance by the last forward, as a
+        numpy array [T, N+2] (Driver::GetTime/GetDt/GetState)."""
+        import numpy as np
+        n = C.c_int64(0)
+        _check(lib().va_trajectory(self._h, instance, None, 0, C.byref(n), _stream(stream)))
+        rows = np.empty((max(n.value, 1), self.problem.n_state + 2), dtype=np.float64)
+        _check(lib().va_trajectory(self._h, instance, rows.ctypes.data, n.value, C.byref(n),
+                                   _stream(stream)))
+        return rows[: n.value]
 
     def max_steps_seen(self, stream=None) -> int:
         v = C.c_int32(0)

@@ -21,6 +21,9 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvectorizedadjoint_amd.so")
+EXAMPLES_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "va_examples.cpp")
+EXAMPLES_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "va_examples")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("VA_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: only explicit fma() calls are fused, so the IEEE operation
@@ -91,12 +94,32 @@ def build(incremental: bool = False, jobs: int | None = None, verbose: bool = Tr
     return LIB
 
 
+def build_examples(verbose: bool = True) -> str:
+    """Compile the drop-in C++ example drivers (tests/cpp/va_examples.cpp) with the
+    host compiler against the C++ headers and the built library: the headers are
+    plain C++17 over the C ABI and the HIP runtime API, no device code."""
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    libdir = os.path.relpath(HERE, os.path.dirname(EXAMPLES_BIN))
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
+           "-I", os.path.join(ROCM, "include"), EXAMPLES_SRC, "-o", EXAMPLES_BIN,
+           "-L", HERE, "-lvectorizedadjoint_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+           f"-Wl,-rpath,$ORIGIN/{libdir}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+    t = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {EXAMPLES_BIN} in {time.time() - t:.1f}s", flush=True)
+    return EXAMPLES_BIN
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--incremental", action="store_true")
     ap.add_argument("-j", "--jobs", type=int, default=None)
     a = ap.parse_args(argv)
     build(incremental=a.incremental, jobs=a.jobs)
+    build_examples()
     return 0
 
 

@@ -73,6 +73,18 @@ __global__ void k_identity_seeds(double *lambda, int n, int64_t B)
     lambda[idx] = (o == i) ? 1.0 : 0.0;
 }
 
+// Plain-form rows are [row][field][instance]; gather (t, h, y) of one instance.
+__global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fields, int n,
+                              int64_t nrows, double *out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = nrows * (n + 2);
+    if (idx >= total) return;
+    const int64_t row = idx / (n + 2);
+    const int f = (int)(idx % (n + 2));
+    out[idx] = ptraj[((size_t)row * fields + f) * (size_t)B + b];
+}
+
 inline int blocks_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace
@@ -527,6 +539,44 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
             return st;
     }
     return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
+va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
+                        int64_t *n_rows, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (!h->have_forward) return fail(VA_ERR_STATE, "va_trajectory needs a va_forward first");
+    if (!n_rows || instance < 0 || instance >= h->pb.batch || max_rows < 0 ||
+        (max_rows > 0 && !rows))
+        return fail(VA_ERR_INVALID, "bad instance, rows or n_rows");
+    hipStream_t s = (hipStream_t)stream;
+    int32_t cnt = 0, lane_st = 0;
+    VA_HIP(hipMemcpyAsync(&cnt, h->n_acc + instance, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    VA_HIP(hipMemcpyAsync(&lane_st, h->status + instance, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          s));
+    VA_HIP(hipStreamSynchronize(s));
+    const int64_t stored = lane_st == VA_LANE_OK ? (int64_t)cnt + 1 : (int64_t)cnt;
+    *n_rows = stored;
+    const int64_t ncopy = std::min<int64_t>(stored, max_rows);
+    if (ncopy <= 0) return VA_OK;
+    const int n = h->ks.n;
+    const size_t bytes = sizeof(double) * (size_t)ncopy * (n + 2);
+    if (h->pb.mode == VA_MODE_PERSISTENT) {
+        const double *src = h->traj + (size_t)instance * (size_t)(h->pb.max_steps + 1) * (n + 2);
+        VA_HIP(hipMemcpyAsync(rows, src, bytes, hipMemcpyDeviceToHost, s));
+    } else {
+        double *tmp = nullptr;
+        VA_HIP(hipMallocAsync((void **)&tmp, bytes, s));
+        const int fields = 2 + n + h->ks.s_rev * n;
+        hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(ncopy * (n + 2), 256)), dim3(256), 0, s,
+                           h->ptraj, instance, h->pb.batch, fields, n, ncopy, tmp);
+        hipError_t e = hipMemcpyAsync(rows, tmp, bytes, hipMemcpyDeviceToHost, s);
+        hipFreeAsync(tmp, s);
+        VA_HIP(e);
+    }
+    VA_HIP(hipStreamSynchronize(s));
+    return VA_OK;
 }
 
 va_status va_max_steps_seen(va_handle *h, int32_t *out, void *stream)
