@@ -141,7 +141,8 @@ def main(argv=None) -> int:
         tf, tr, n_acc, n_rej = kernel_times(s, y0, p, 5, torch)
     n_ok = int((st == 0).sum())
     fl_f = FL.forward_flops(w.system, w.n, w.tableau, n_acc, n_rej)
-    fl_r = FL.reverse_flops(w.system, w.n, w.tableau, n_acc)
+    # the reverse sweeps only instances whose forward ended OK
+    fl_r = FL.reverse_flops(w.system, w.n, w.tableau, n_acc[st == 0])
     dom = ("reverse", tr, fl_r) if tr >= tf else ("forward", tf, fl_f)
     achieved = dom[2] / (dom[1] * 1e-3) / 1e12
 

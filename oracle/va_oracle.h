@@ -30,6 +30,26 @@
  * for bit.  The controller's pow() is orc_pow(): natural log by the atanh
  * series and exp by Cody-Waite reduction plus a Taylor polynomial (see
  * va_oracle.c), within a few ulp of the C library pow.
+ *
+ * How each part is pinned (tests/ file: what fixes it independently):
+ *   tableaus ........ test_oracle_tableau: rooted-tree order conditions to the
+ *                     stated order, row sums, DOPRI5 FSAL row, observed
+ *                     convergence order of the fixed-step loop
+ *   rhs / vjp ....... test_oracle_math: the reference functors' closed forms,
+ *                     complex-step derivatives; the alphasfile fixtures' layout
+ *   forward ......... test_oracle_forward: oscillator closed form, SciPy DOP853
+ *                     at rtol 1e-13 for Van der Pol and Lotka-Volterra, fixed
+ *                     grid t0 + n dt, FSAL evaluation count, status paths
+ *   controller ...... test_oracle_forward::test_controller_invariants (accepted
+ *                     err <= 1, shrink in [0.2, 1), growth bounds, dead band,
+ *                     landing on t1) and the shrink-exponent mutant
+ *   adjoint ......... test_oracle_adjoint: complex-step derivative of the frozen
+ *                     step map, oscillator transfer matrix, central differences,
+ *                     the reference's seed semantics, bitwise replay
+ *   mutants ......... test_oracle_mutants: five planted mistakes, each caught
+ * Parity unpinned: the exact accept/reject decisions of Boost.odeint 1.74
+ * itself (Boost is not installed, and the reference holds no recorded
+ * outputs); the restated controller is pinned only by the invariants above.
  */
 #ifndef VA_ORACLE_H
 #define VA_ORACLE_H

@@ -11,6 +11,7 @@
 //           [0, 10], dt0 = 1e-3, Nout = N identity seeds; plus the sensitivity
 //           matrix.
 //   pre  -- the precondition messages of adjointSolve (backpropagation.hpp:23-42).
+//   fn   -- the batched AdjointFunction (forward / backward over 64 instances).
 //
 // Usage: va_examples <scenario> [tol] [N] [alphas.csv]
 // Prints one JSON object with the results (all doubles to 17 digits).
@@ -188,12 +189,59 @@ int preconditions()
     return 0;
 }
 
+// AdjointFunction over a batch: Van der Pol, DOPRI5 tol 1e-8 on [0, 20], B = 64,
+// y0 = (2 + 0.01 s_b, 0), mu_b = 0.5 + 4.5 (b + 0.5) / B, gy = 2 y(T)
+// (the gradient of |y(T)|^2).
+int batched()
+{
+    const int B = 64;
+    va_problem pb;
+    va_problem_init(&pb);
+    pb.system = VA_SYSTEM_VAN_DER_POL;
+    pb.tableau = VA_TABLEAU_DOPRI5;
+    pb.n_state = 2;
+    pb.n_param = 1;
+    pb.batch = B;
+    pb.t1 = 20.0;
+    pb.abs_tol = pb.rel_tol = 1e-8;
+    std::vector<double> y(2 * B), p(B);
+    for (int b = 0; b < B; ++b) {
+        y[b] = 2.0 + 0.01 * ((b % 7) - 3);
+        y[B + b] = 0.0;
+        p[b] = 0.5 + 4.5 * (b + 0.5) / B;
+    }
+    const std::vector<double> y0 = y;
+    detail_driver::device_array<double> dy, dp, dg, dgy0, dgp;
+    dy.upload(y);
+    dp.upload(p);
+    AdjointFunction fn(pb);
+    fn.forward(dy.ptr, dp.ptr);
+    dy.download(y.data(), y.size());
+    std::vector<double> gy(2 * B);
+    for (int i = 0; i < 2 * B; ++i) gy[i] = 2.0 * y[i];
+    dg.upload(gy);
+    dgy0.reserve(2 * B);
+    dgp.reserve(B);
+    fn.backward(dp.ptr, dg.ptr, nullptr, dgy0.ptr, dgp.ptr);
+    std::vector<double> gy0(2 * B), gp(B);
+    dgy0.download(gy0.data(), gy0.size());
+    dgp.download(gp.data(), gp.size());
+    std::printf("{\"scenario\": \"fn\", \"batch\": %d,\n", B);
+    put("y0", y0);
+    put("p", p);
+    put("yT", y);
+    put("dJdy0", gy0);
+    put("dJdp", gp, false);
+    std::printf("}\n");
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::cerr << "usage: " << argv[0] << " vdp|ho|glv|pre [tol] [N] [alphas.csv]\n";
+        std::cerr << "usage: " << argv[0] << " vdp|ho|glv|pre|fn [tol] [N] [alphas.csv]\n";
         return 1;
     }
     const std::string which = argv[1];
@@ -220,6 +268,8 @@ int main(int argc, char **argv)
                 LotkaVolterra(), state(N, 0.1), alphas, 0.0, 10.0, 1e-3, 1, true);
         } else if (which == "pre") {
             return preconditions();
+        } else if (which == "fn") {
+            return batched();
         } else {
             std::cerr << "unknown scenario " << which << "\n";
             return 1;

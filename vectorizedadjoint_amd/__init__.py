@@ -166,6 +166,7 @@ class Solver:
         self.problem = problem
         self._h = C.c_void_p()
         self._pb = problem.to_c()
+        self.generation = 0  # forwards run so far; the reverse uses the last one's trajectory
         _check(lib().va_create(C.byref(self._pb), C.byref(self._h)))
 
     def close(self) -> None:
@@ -203,6 +204,7 @@ class Solver:
             self._shape_check(t, (B,), "i32")
         _check(lib().va_forward(self._h, _ptr(y), _ptr(p), _ptr(n_accepted), _ptr(n_rejected),
                                 _ptr(lane_status), _stream(stream)))
+        self.generation += 1
         return y
 
     def adjoint(self, p, lam, mu, stream=None):
@@ -245,6 +247,7 @@ class Solver:
                               _ptr(out["J"]), _ptr(out["yT"]), _ptr(out["dJdy0"]),
                               _ptr(out["dJdp"]), _ptr(out.get("n_acc")), _ptr(out.get("n_rej")),
                               _ptr(out.get("status")), _stream(stream)))
+        self.generation += 1
         return out
 
     def trajectory(self, instance: int, stream=None):
