@@ -85,6 +85,66 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
     out[idx] = ptraj[((size_t)row * fields + f) * (size_t)B + b];
 }
 
+// Longest-first order of the reverse sweep.  The reverse of one instance is a
+// chain of n_acc dependent steps; when the long chains start last they set the
+// end of the launch.  Instances are bucketed by accepted-step count
+// (VA_ORDER_BUCKETS buckets of 16 steps, the last open-ended) and the reverse
+// takes them bucket by bucket from the longest; within a bucket the order is
+// arbitrary.  Each instance's result does not depend on the order.
+constexpr int VA_ORDER_BUCKETS = 64;
+
+__device__ __forceinline__ int order_bucket(int32_t n_acc, int32_t status)
+{
+    if (status != 0) return 0;  // failed instances only write NaN: last
+    const int k = n_acc >> 4;
+    return k < VA_ORDER_BUCKETS - 1 ? k : VA_ORDER_BUCKETS - 1;
+}
+
+__global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const int32_t *status,
+                                                    int64_t B, int32_t *hist)
+{
+    __shared__ int32_t cnt[VA_ORDER_BUCKETS];
+    if (threadIdx.x < VA_ORDER_BUCKETS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) atomicAdd(&cnt[order_bucket(n_acc[b], status[b])], 1);
+    __syncthreads();
+    if (threadIdx.x < VA_ORDER_BUCKETS && cnt[threadIdx.x])
+        atomicAdd(&hist[threadIdx.x], cnt[threadIdx.x]);
+}
+
+// cursor[k] = number of instances in buckets above k (longest bucket first).
+__global__ void k_order_offsets(const int32_t *hist, int32_t *cursor)
+{
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int k = VA_ORDER_BUCKETS - 1; k >= 0; --k) {
+            cursor[k] = acc;
+            acc += hist[k];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_order_scatter(const int32_t *n_acc,
+                                                       const int32_t *status, int64_t B,
+                                                       int32_t *cursor, int32_t *order)
+{
+    __shared__ int32_t cnt[VA_ORDER_BUCKETS], base[VA_ORDER_BUCKETS];
+    if (threadIdx.x < VA_ORDER_BUCKETS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k = 0, r = 0;
+    if (b < B) {
+        k = order_bucket(n_acc[b], status[b]);
+        r = atomicAdd(&cnt[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < VA_ORDER_BUCKETS && cnt[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (b < B) order[base[k] + r] = (int32_t)b;
+}
+
 inline int blocks_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace
@@ -105,6 +165,8 @@ struct va_handle {
     // bookkeeping [B] and counters
     int32_t *n_acc = nullptr, *n_rej = nullptr, *status = nullptr;
     int32_t *counters = nullptr;  // [0] fwd queue, [1] rev queue, [2] max steps, [3] active
+    int32_t *order = nullptr;     // [B] reverse order, longest first
+    int32_t *order_tmp = nullptr; // [2 * VA_ORDER_BUCKETS] histogram, cursors
     int32_t *host_counter = nullptr;
     bool have_forward = false;
 };
@@ -165,6 +227,8 @@ void release(va_handle *h)
     hipFree(h->n_rej);
     hipFree(h->status);
     hipFree(h->counters);
+    hipFree(h->order);
+    hipFree(h->order_tmp);
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
@@ -196,8 +260,19 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->counters + 1, 0, 1, s);
     if (st) return st;
+    // longest-first order over the instances of the last forward
+    if ((st = fill_i32(h->order_tmp, 0, 2 * VA_ORDER_BUCKETS, s))) return st;
+    const int gb = blocks_for(B, 256);
+    hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
+                       h->order_tmp);
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
+                       h->order_tmp + VA_ORDER_BUCKETS);
+    hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
+                       h->order_tmp + VA_ORDER_BUCKETS, h->order);
+    VA_HIP(hipGetLastError());
     va::RevArgs a;
     a.sp = solve_params(h);
+    a.order = h->order;
     a.p = p;
     a.traj = h->traj;
     a.n_acc = h->n_acc;
@@ -414,6 +489,8 @@ va_status va_create(const va_problem *pb, va_handle **out)
     alloc((void **)&h->n_rej, sizeof(int32_t) * B);
     alloc((void **)&h->status, sizeof(int32_t) * B);
     alloc((void **)&h->counters, sizeof(int32_t) * 4);
+    alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
+    alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
     if (pb->mode == VA_MODE_PERSISTENT) {
         alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {

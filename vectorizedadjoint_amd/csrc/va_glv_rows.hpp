@@ -95,6 +95,21 @@ __device__ __forceinline__ void exchange(double *slab, int lane, int gbase, doub
     __builtin_amdgcn_wave_barrier();
 }
 
+// Two exchanges through two slabs with one pair of wave barriers, so the
+// arithmetic on both results can be interleaved by the scheduler.
+template <int NN>
+__device__ __forceinline__ void exchange2(double *slab_a, double *slab_b, int lane, int gbase,
+                                          double va, double (&out_a)[NN], double vb,
+                                          double (&out_b)[NN])
+{
+    slab_a[lane] = va;
+    slab_b[lane] = vb;
+    __builtin_amdgcn_wave_barrier();
+    gather_vec<NN>(slab_a, gbase, out_a);
+    gather_vec<NN>(slab_b, gbase, out_b);
+    __builtin_amdgcn_wave_barrier();
+}
+
 // g_i = r_i + sum_j A_ij x_j: two interleaved partial sums (even j from r_i,
 // odd j from 0) added at the end -- half the dependent-FMA depth, same order as
 // the one-lane kernels and the oracle.
@@ -132,15 +147,22 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
     using L = GlvRows<NN>;
     constexpr int F = L::F;
     __shared__ double2 slab2[L::WAVES][32];
+    __shared__ double2 slabb2[L::WAVES][32];
     __shared__ int gidx[L::WAVES][L::GPW];
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int grp = lane / NN, i = lane - grp * NN;
     if (grp >= L::GPW) return;
     double *slab = reinterpret_cast<double *>(slab2[wv]);
+    double *slabb = reinterpret_cast<double *>(slabb2[wv]);
     const int gbase = grp * NN;
     const SolveParams sp = a.sp;
     const int64_t B = sp.B;
+    // For tableaus without FSAL the next step's f(y_{n+1}) is computed with the
+    // error norm of the attempt (it does not depend on the step decision), so it
+    // is off the accept path's critical chain; it is the same arithmetic as
+    // evaluating it at the start of the next step.
+    constexpr bool kSpec = Adaptive && !Tab::FSAL;
 
     double Arow[NN], r = 0.0, y = 0.0, K[Tab::SF], ynew = 0.0, xerr = 0.0;
     double t = 0.0, dt = 0.0;
@@ -185,7 +207,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
                 if (i == 0) row[0] = t;
                 row[2 + i] = y;
                 if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
-                if (!Tab::FSAL || !have_k0) {
+                if ((!Tab::FSAL && !kSpec) || !have_k0) {
                     exchange<NN>(slab, lane, gbase, y, v);
                     K[0] = y * growth_row<NN>(Arow, r, v);
                     have_k0 = true;
@@ -215,8 +237,14 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
                 for (int j = 0; j < Tab::SF; ++j)
                     if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
                 const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
-                exchange<NN>(slab, lane, gbase, fabs(xerr) / den, v);
-                const double err = group_max<NN>(v);
+                double k0n = 0.0, ve[NN];
+                if constexpr (kSpec) {
+                    exchange2<NN>(slab, slabb, lane, gbase, ynew, v, fabs(xerr) / den, ve);
+                    k0n = ynew * growth_row<NN>(Arow, r, v);
+                } else {
+                    exchange<NN>(slabb, lane, gbase, fabs(xerr) / den, ve);
+                }
+                const double err = group_max<NN>(ve);
                 double dt_next;
                 if (!control_step<Tab>(dt, err, dt_next)) {
                     ++rej;
@@ -225,6 +253,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
                     t = t + dt;
                     y = ynew;
                     if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
+                    if constexpr (kSpec) K[0] = k0n;
                     ++count;
                     new_step = true;
                 }
@@ -306,6 +335,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
             b = gidx[wv][grp];
             __builtin_amdgcn_wave_barrier();
             if (b >= B) break;
+            if (a.order) b = a.order[b];
             if (a.status[b] != LANE_OK) {
                 const int nseed = a.cost ? 1 : a.nout;
                 for (int oo = 0; oo < nseed; ++oo) {

@@ -42,6 +42,7 @@ struct RevArgs {
     double *mu;            // [nout][P][B]
     int32_t nout;
     int32_t *counter;
+    const int32_t *order;  // [B] or null: the k-th instance taken is order[k]
 };
 
 // Plain form: one launch per accepted step, state in HBM, no refill, full

@@ -239,6 +239,7 @@ __global__ void __launch_bounds__(256) k_rev(RevArgs a)
         if (need) {
             b = atomicAdd(a.counter, 1);
             if (b >= B) break;
+            if (a.order) b = a.order[b];
             if (a.status[b] != LANE_OK) {
                 const int nseed = a.cost ? 1 : a.nout;
                 for (int oo = 0; oo < nseed; ++oo) {
