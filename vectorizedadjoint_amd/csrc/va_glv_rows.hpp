@@ -85,6 +85,14 @@ __device__ __forceinline__ void prioritise_long(bool long_running)
 
 constexpr int VA_LONG_ATTEMPTS = 160;  // ~2x the median attempts of an instance
 
+// Wait for this wave's outstanding vector-memory operations.  Placed at the end
+// of a refill, it retires the instance's parameter loads there; otherwise the
+// compiler cannot tell at the loop head whether a load is pending and waits
+// for every outstanding operation -- including the trajectory stores, which
+// count in the same in-order counter -- on each trip.  (gfx9 s_waitcnt
+// encoding: vmcnt 0, expcnt and lgkmcnt left at their maxima.)
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <int NN>
 __device__ __forceinline__ void exchange(double *slab, int lane, int gbase, double v,
                                          double (&out)[NN])
@@ -192,6 +200,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
             have_k0 = false;
             need = false;
             rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+            wait_vmem();
         }
         bool finish = false;
         if (new_step) {
@@ -321,6 +330,8 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
     const double qnan = __builtin_nan("");
 
     double Arow[NN], Acol[NN], Abar[NN], r = 0.0, rbar = 0.0, lam = 0.0;
+    // row n's (h_n, y_n) of this lane, loaded one step ahead of its use
+    double h_cur = 0.0, y_cur = 0.0;
     int64_t b = 0;
     int32_t o = 0, n = -1;
     bool need = true;
@@ -377,12 +388,21 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 #pragma unroll
             for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
             n = a.n_acc[b] - 1;
+            if (n >= 0) {
+                h_cur = rows[(size_t)n * F + 1];
+                y_cur = rows[(size_t)n * F + 2 + i];
+            }
             need = false;
+            wait_vmem();
         }
         if (n >= 0) {
-            const double *row = rows + (size_t)n * F;
-            const double h = row[1];
-            const double yn = row[2 + i];
+            const double h = h_cur;
+            const double yn = y_cur;
+            if (n > 0) {
+                const double *prev = rows + (size_t)(n - 1) * F;
+                h_cur = prev[1];
+                y_cur = prev[2 + i];
+            }
             double X[S], g[S], K[S], Kb[S];
             // recompute the stages from y_n (backpropagation.hpp:24-64)
 #pragma unroll
@@ -445,6 +465,11 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 #pragma unroll
                 for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
                 n = a.n_acc[b] - 1;
+                if (n >= 0) {
+                    h_cur = rows[(size_t)n * F + 1];
+                    y_cur = rows[(size_t)n * F + 2 + i];
+                }
+                wait_vmem();
             }
         }
     }
