@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define VA_ABI_VERSION 1
+#define VA_ABI_VERSION 2
 
 typedef enum va_status {
     VA_OK = 0,
@@ -102,11 +102,16 @@ typedef struct va_problem {
     int32_t block_size;        /* threads per block (64..256, multiple of 64), 0 = tuned */
     int32_t instances_per_wave;  /* 64: one lane per instance; 64/N: one group of N
                                     lanes per instance (Lotka-Volterra); 0 = tuned    */
-    int32_t checkpoint_interval; /* reserved, must be 0                               */
+    int32_t checkpoint_slots;  /* observed solves: 0 = store every step while the store
+                                  fits trajectory_budget_bytes, else checkpoint; > 0 =
+                                  checkpoint with this many slots per instance (even) */
     int64_t lanes;             /* instances in flight in the persistent kernels, 0 = all
                                   resident slots; fewer than B means lanes are refilled */
     int64_t trajectory_budget_bytes; /* 0 = unlimited; va_create fails if the store
+                                        (or, for observed solves, the checkpoint store)
                                         would exceed it                               */
+    int32_t n_obs;             /* observation times of va_solve_observed (0 = none)   */
+    int32_t reserved;          /* must be 0                                           */
 } va_problem;
 
 typedef struct va_handle va_handle;
@@ -153,6 +158,23 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
                    const double *cost_weights, double *J, double *yT, double *dJdy0,
                    double *dJdp, int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
                    void *stream);
+
+/* Forward + observation cost + adjoint (no reference counterpart: the reference's
+ * cost is terminal only, examples/VanDerPol/main.cpp:98-117).  The controlled loop stops at each
+ * observation time obs_t[k] (device, [n_obs], increasing, inside [t0, t1]) the way
+ * it stops at t1 (detail/runge_kutta.hpp:98-101); after such a shortened step the
+ * next step size is the larger of the controller's proposal and the step size
+ * before shortening (Boost.odeint integrate_times).  J [B] = sum_k |y(t_k) -
+ * obs_y[k]|^2 with obs_y [n_obs][N] (device, shared by the batch); dJdy0 [N][B],
+ * dJdp [P][B] overwritten; yT [N][B].  When the trajectory store does not fit
+ * trajectory_budget_bytes the forward keeps checkpoints instead (a fixed number of
+ * checkpoint slots per instance whose spacing doubles as the instance runs long)
+ * and the reverse recomputes each segment from its checkpoint, bit-identically.  Adaptive
+ * one-lane systems only (Van der Pol, oscillators); needs pb.n_obs > 0. */
+va_status va_solve_observed(va_handle *h, const double *y0, const double *p, const double *obs_t,
+                            const double *obs_y, double *J, double *yT, double *dJdy0,
+                            double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                            int32_t *lane_status, void *stream);
 
 /* Driver::GetT/GetTime/GetDt/GetState (lib/include/Driver.hpp:53-65,
  * StateStorage.hpp:17-42): copies the rows stored for one instance by the last

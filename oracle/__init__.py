@@ -62,6 +62,22 @@ class Trajectory(C.Structure):
                 ("h", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double))]
 
 
+class Obs(C.Structure):
+    _fields_ = [("k", C.c_int), ("t", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double))]
+
+
+def observations(t_obs, y_obs):
+    """An Obs structure over numpy arrays t_obs [K], y_obs [K, N] (kept alive on it)."""
+    t_obs = np.ascontiguousarray(t_obs, dtype=np.float64)
+    y_obs = np.ascontiguousarray(y_obs, dtype=np.float64)
+    ob = Obs()
+    ob.k = int(t_obs.size)
+    ob.t = _dp(t_obs)
+    ob.y = _dp(y_obs)
+    ob._keep = (t_obs, y_obs)
+    return ob
+
+
 class Stats(C.Structure):
     _fields_ = [("accepted", C.c_int32), ("rejected", C.c_int32),
                 ("rhs_evals", C.c_int32), ("status", C.c_int32)]
@@ -105,6 +121,15 @@ def lib():
                                       dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp,
                                       ip, ip, ip, C.c_int]
         L.orc_solve_batch.restype = C.c_int
+        L.orc_forward_obs.argtypes = [C.POINTER(Problem), C.POINTER(Obs), dp, dp,
+                                      C.POINTER(Trajectory), C.POINTER(Stats), ip, dp]
+        L.orc_forward_obs.restype = C.c_int64
+        L.orc_backward_obs.argtypes = [C.POINTER(Problem), C.POINTER(Obs), dp,
+                                       C.POINTER(Trajectory), ip, dp, dp]
+        L.orc_solve_batch_obs.argtypes = [C.POINTER(Problem), C.POINTER(Obs), C.c_int64,
+                                          C.c_int64, C.c_int64, dp, dp, dp, dp, dp, dp, ip, ip,
+                                          ip, C.c_int]
+        L.orc_solve_batch_obs.restype = C.c_int
         _lib = L
     return _lib
 
@@ -207,7 +232,7 @@ def rk_step(pb, y, p, t, h):
 class Solve:
     """One instance: forward (with stored trajectory) and adjoint."""
 
-    def __init__(self, pb, y0, p, log_cap=0):
+    def __init__(self, pb, y0, p, log_cap=0, obs=None):
         self.pb = pb
         self.p = np.ascontiguousarray(p, dtype=np.float64)
         self.y = np.array(y0, dtype=np.float64)
@@ -218,8 +243,19 @@ class Solve:
         self.st = Stats()
         self.log = np.zeros((max(log_cap, 1), 4))
         nlog = C.c_int64(0)
-        self.steps = lib().orc_forward(C.byref(pb), _dp(self.y), _dp(self.p), C.byref(self.tr),
-                                       C.byref(self.st), _dp(self.log), log_cap, C.byref(nlog))
+        self.obs = None
+        if obs is not None:
+            self.obs = observations(*obs)
+            self.obs_row = np.zeros(self.obs.k, np.int32)
+            J = C.c_double(0.0)
+            self.steps = lib().orc_forward_obs(C.byref(pb), C.byref(self.obs), _dp(self.y),
+                                               _dp(self.p), C.byref(self.tr), C.byref(self.st),
+                                               _ip(self.obs_row), C.byref(J))
+            self.J = J.value
+        else:
+            self.steps = lib().orc_forward(C.byref(pb), _dp(self.y), _dp(self.p),
+                                           C.byref(self.tr), C.byref(self.st), _dp(self.log),
+                                           log_cap, C.byref(nlog))
         self.log = self.log[: nlog.value]
 
     def __del__(self):
@@ -252,6 +288,14 @@ class Solve:
         lib().orc_backward(C.byref(self.pb), _dp(self.p), C.byref(self.tr), _dp(lam), _dp(mu))
         return lam, mu
 
+    def backward_obs(self, lam=None):
+        """Gradient of the observation cost: (dJ/dy0, dJ/dp)."""
+        lam = np.zeros(self.pb.n) if lam is None else np.array(lam, dtype=np.float64)
+        mu = np.zeros(self.pb.np)
+        lib().orc_backward_obs(C.byref(self.pb), C.byref(self.obs), _dp(self.p),
+                               C.byref(self.tr), _ip(self.obs_row), _dp(lam), _dp(mu))
+        return lam, mu
+
     def adjoint_solve(self, lam, mu):
         lam = np.ascontiguousarray(lam, dtype=np.float64).copy()
         mu = np.ascontiguousarray(mu, dtype=np.float64).copy()
@@ -274,8 +318,9 @@ class Solve:
 
 
 def solve_batch(pb, y0, p, cost, cost_w=None, p_shared=None, b_begin=0, b_count=None,
-                nthreads=None):
-    """SoA batch: y0 [N][B], p [P][B].  Returns a dict of SoA outputs."""
+                nthreads=None, obs=None):
+    """SoA batch: y0 [N][B], p [P][B].  Returns a dict of SoA outputs.
+    obs=(t_obs [K], y_obs [K, N]): the observation cost replaces `cost`."""
     y0 = np.ascontiguousarray(y0, dtype=np.float64)
     p = np.ascontiguousarray(p, dtype=np.float64)
     N, B = y0.shape
@@ -293,6 +338,15 @@ def solve_batch(pb, y0, p, cost, cost_w=None, p_shared=None, b_begin=0, b_count=
         out["dJdA"] = np.zeros(N * N)
         p_shared = np.ascontiguousarray(p_shared, dtype=np.float64)
     cw = None if cost_w is None else np.ascontiguousarray(cost_w, dtype=np.float64)
+    if obs is not None:
+        ob = observations(*obs)
+        rc = lib().orc_solve_batch_obs(
+            C.byref(pb), C.byref(ob), B, b_begin, b_count, _dp(y0), _dp(p), _dp(out["J"]),
+            _dp(out["yT"]), _dp(out["dJdy0"]), _dp(out["dJdp"]), _ip(out["n_acc"]),
+            _ip(out["n_rej"]), _ip(out["status"]), nthreads)
+        if rc != 0:
+            raise ValueError("orc_solve_batch_obs rejected its arguments")
+        return out
     rc = lib().orc_solve_batch(
         C.byref(pb), B, b_begin, b_count, _dp(y0), _dp(p), _dp(p_shared) if shared else None,
         cost, _dp(cw), _dp(out["J"]), _dp(out["yT"]), _dp(out["dJdy0"]), _dp(out["dJdp"]),

@@ -497,8 +497,34 @@ static int all_finite(int n, const double *y)
     return 1;
 }
 
+static int64_t forward_impl(const orc_problem *pb, const orc_obs *ob, double *y, const double *p,
+                            orc_trajectory *tr, orc_stats *st, double *log, int64_t log_cap,
+                            int64_t *log_len, int32_t *obs_row, double *Jobs);
+
 int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_trajectory *tr,
                     orc_stats *st, double *log, int64_t log_cap, int64_t *log_len)
+{
+    return forward_impl(pb, NULL, y, p, tr, st, log, log_cap, log_len, NULL, NULL);
+}
+
+int64_t orc_forward_obs(const orc_problem *pb, const orc_obs *ob, double *y, const double *p,
+                        orc_trajectory *tr, orc_stats *st, int32_t *obs_row, double *J)
+{
+    return forward_impl(pb, ob, y, p, tr, st, NULL, 0, NULL, obs_row, J);
+}
+
+/* Forward solve; with observations (ob != NULL, controlled loop only) the loop
+ * runs segment by segment to each observation time and then to t1.  A step
+ * that would pass the next observation time is shortened to land on it, as at
+ * t1 (detail/runge_kutta.hpp:98-101).  After such a shortened step is accepted
+ * the next step size is the larger of the controller's proposal and the step
+ * size before shortening -- Boost.odeint 1.74 integrate_times for controlled
+ * steppers does the same (third party, restated).  At each observation time
+ * the running cost J += sum_i (y_i - y_obs_k,i)^2 and obs_row[k] = the index
+ * of the stored point (the accepted steps so far). */
+static int64_t forward_impl(const orc_problem *pb, const orc_obs *ob, double *y, const double *p,
+                            orc_trajectory *tr, orc_stats *st, double *log, int64_t log_cap,
+                            int64_t *log_len, int32_t *obs_row, double *Jobs)
 {
     orc_tableau tb;
     orc_work w;
@@ -506,8 +532,11 @@ int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_traje
     orc_stats s0 = {0, 0, 0, ORC_OK};
     int64_t nlog = 0;
     if (tr) tr->T = 0;
+    const int nobs = ob ? ob->k : 0;
+    double Jacc = 0.0;
+    if (Jobs) *Jobs = 0.0;
     if (orc_get_tableau(pb->tableau, &tb) != 0 || n <= 0 || n > ORC_MAX_N ||
-        (pb->controlled && !tb.has_err)) {
+        (pb->controlled && !tb.has_err) || (nobs > 0 && !pb->controlled)) {
         s0.status = ORC_BAD_INPUT;
         if (st) *st = s0;
         return 0;
@@ -538,12 +567,20 @@ int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_traje
 
     /* detail/runge_kutta.hpp:74-118 (controlled_stepper_tag) */
     int have_k0 = 0;
-    while (less_with_sign(t, t1, dt)) {
+    for (int kobs = 0;; ++kobs) {
+    const int is_obs = kobs < nobs;
+    const double target = is_obs ? ob->t[kobs] : t1;
+    while (less_with_sign(t, target, dt)) {
         if (record(tr, n, t, y) != 0 || (pb->max_steps > 0 && count >= pb->max_steps)) {
             s0.status = ORC_MAX_STEPS;
             goto done;
         }
-        if (less_with_sign(t1, t + dt, dt)) dt = t1 - t;
+        int clipped = 0;
+        const double dt_keep = dt;
+        if (less_with_sign(target, t + dt, dt)) {
+            dt = target - t;
+            clipped = is_obs;
+        }
         int tries = 0;
         int ok = 0;
         /* odeint evaluates f(y) on every try of a non-FSAL stepper; the value
@@ -562,11 +599,13 @@ int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_traje
             if (err > 1.0) {
                 dt = decrease_step(dt, err, tb.err_order);
                 s0.rejected += 1;
+                clipped = 0;
                 ok = 0;
             } else {
                 if (tr) tr->h[tr->T - 1] = dt;
                 t = t + dt;
                 dt = increase_step(dt, err, tb.order);
+                if (clipped && fabs(dt_keep) > fabs(dt)) dt = dt_keep;
                 memcpy(y, w.ynew, sizeof(double) * (size_t)n);
                 if (tb.fsal) memcpy(w.K[0], w.K[tb.s_fwd - 1], sizeof(double) * (size_t)n);
                 ok = 1;
@@ -588,7 +627,15 @@ int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_traje
         ++count;
         s0.accepted = (int32_t)count;
     }
+    if (!is_obs) break;
+    if (obs_row) obs_row[kobs] = (int32_t)count;
+    for (int i = 0; i < n; ++i) {
+        const double d = y[i] - ob->y[(size_t)kobs * n + i];
+        Jacc = fma(d, d, Jacc);
+    }
+    }
     if (record(tr, n, t, y) != 0) s0.status = ORC_MAX_STEPS;
+    if (Jobs) *Jobs = Jacc;
 
 done:
     if (s0.status == ORC_OK && !all_finite(n, y)) s0.status = ORC_NONFINITE;
@@ -686,6 +733,44 @@ void orc_backward(const orc_problem *pb, const double *p, const orc_trajectory *
     free(w);
 }
 
+/* lambda += dJ_k/dy = 2 (y_n - y_obs_k) for every observation taken at point n. */
+static int obs_jumps(const orc_obs *ob, int n, const int32_t *obs_row, int kobs, int64_t point,
+                     const double *yn, double *lambda)
+{
+    while (kobs >= 0 && obs_row[kobs] == point) {
+        for (int i = 0; i < n; ++i)
+#if ORC_MUTANT == 6 /* planted sign error, test builds only */
+            lambda[i] = fma(-2.0, yn[i] - ob->y[(size_t)kobs * n + i], lambda[i]);
+#else
+            lambda[i] = fma(2.0, yn[i] - ob->y[(size_t)kobs * n + i], lambda[i]);
+#endif
+        --kobs;
+    }
+    return kobs;
+}
+
+void orc_backward_obs(const orc_problem *pb, const orc_obs *ob, const double *p,
+                      const orc_trajectory *tr, const int32_t *obs_row, double *lambda,
+                      double *mu)
+{
+    orc_tableau tb;
+    orc_get_tableau(pb->tableau, &tb);
+    int n = pb->n, np = pb->np;
+    orc_rwork *w = (orc_rwork *)malloc(sizeof(orc_rwork));
+    double *pbar = (double *)calloc((size_t)np, sizeof(double));
+    double *wjp = (double *)calloc((size_t)np, sizeof(double));
+    int kobs = ob->k - 1;
+    kobs = obs_jumps(ob, n, obs_row, kobs, tr->T - 1, tr->y + (size_t)(tr->T - 1) * n, lambda);
+    for (int64_t k = tr->T - 2; k >= 0; --k) {
+        back_prop_step(pb, &tb, p, tr, k, lambda, pbar, wjp, w);
+        kobs = obs_jumps(ob, n, obs_row, kobs, k, tr->y + (size_t)k * n, lambda);
+    }
+    for (int k = 0; k < np; ++k) mu[k] += pbar[k];
+    free(wjp);
+    free(pbar);
+    free(w);
+}
+
 void orc_adjoint_solve(const orc_problem *pb, const double *p, const orc_trajectory *tr,
                        int nout, double *lambda, double *mu)
 {
@@ -721,6 +806,7 @@ typedef struct {
     pthread_mutex_t mtx;
     double *shared_acc;   /* sum of dJ/dA over instances */
     int np_full;
+    const orc_obs *ob;    /* observation cost instead of the terminal cost */
 } batch_ctx;
 
 static void *batch_worker(void *arg)
@@ -739,6 +825,7 @@ static void *batch_worker(void *arg)
     double *mu = (double *)malloc(sizeof(double) * (size_t)npf);
     double *acc = shared ? (double *)calloc((size_t)n * n, sizeof(double)) : NULL;
     double y[ORC_MAX_N], lam[ORC_MAX_N];
+    int32_t *obs_row = c->ob ? (int32_t *)malloc(sizeof(int32_t) * (size_t)(c->ob->k + 1)) : NULL;
     const int64_t chunk = 8;
     for (;;) {
         int64_t b0 = __atomic_fetch_add(&c->next, chunk, __ATOMIC_RELAXED);
@@ -754,24 +841,35 @@ static void *batch_worker(void *arg)
                 for (int k = 0; k < npf; ++k) pp[k] = c->p[(size_t)k * B + b];
             }
             orc_stats st;
-            orc_forward(&lp, y, pp, &tr, &st, NULL, 0, NULL);
+            double Jo = 0.0;
+            if (c->ob)
+                orc_forward_obs(&lp, c->ob, y, pp, &tr, &st, obs_row, &Jo);
+            else
+                orc_forward(&lp, y, pp, &tr, &st, NULL, 0, NULL);
             if (c->n_acc) c->n_acc[b] = st.accepted;
             if (c->n_rej) c->n_rej[b] = st.rejected;
             if (c->status) c->status[b] = st.status;
             int okf = st.status == ORC_OK;
             double Jv = 0.0;
-            for (int i = 0; i < n; ++i) {
+            for (int i = 0; i < n && !c->ob; ++i) {
                 switch (c->cost) {
                 case ORC_COST_SUM: Jv += y[i]; lam[i] = 1.0; break;
                 case ORC_COST_SQUARED_NORM: Jv = fma(y[i], y[i], Jv); lam[i] = 2.0 * y[i]; break;
                 default: Jv = fma(c->cost_w[i], y[i], Jv); lam[i] = c->cost_w[i]; break;
                 }
             }
+            if (c->ob) {
+                Jv = Jo;
+                for (int i = 0; i < n; ++i) lam[i] = 0.0;
+            }
             if (c->J) c->J[b] = okf ? Jv : NAN;
             if (c->yT)
                 for (int i = 0; i < n; ++i) c->yT[(size_t)i * B + b] = y[i];
             for (int k = 0; k < npf; ++k) mu[k] = 0.0;
-            if (okf) orc_backward(&lp, pp, &tr, lam, mu);
+            if (okf && c->ob)
+                orc_backward_obs(&lp, c->ob, pp, &tr, obs_row, lam, mu);
+            else if (okf)
+                orc_backward(&lp, pp, &tr, lam, mu);
             if (c->dJdy0)
                 for (int i = 0; i < n; ++i) c->dJdy0[(size_t)i * B + b] = okf ? lam[i] : NAN;
             if (c->dJdp) {
@@ -787,6 +885,7 @@ static void *batch_worker(void *arg)
         for (int k = 0; k < n * n; ++k) c->shared_acc[k] += acc[k];
         pthread_mutex_unlock(&c->mtx);
     }
+    free(obs_row);
     free(acc);
     free(mu);
     free(pp);
@@ -794,14 +893,42 @@ static void *batch_worker(void *arg)
     return NULL;
 }
 
+static int solve_batch_impl(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
+                            int64_t b_count, const double *y0, const double *p,
+                            const double *p_shared, int cost, const double *cost_w, double *J,
+                            double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
+                            int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads);
+
 int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b_count,
                     const double *y0, const double *p, const double *p_shared,
                     int cost, const double *cost_w,
                     double *J, double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
                     int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads)
 {
+    return solve_batch_impl(pb, NULL, B, b_begin, b_count, y0, p, p_shared, cost, cost_w, J, yT,
+                            dJdy0, dJdp, dJdp_shared, n_acc, n_rej, status, nthreads);
+}
+
+int orc_solve_batch_obs(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
+                        int64_t b_count, const double *y0, const double *p, double *J,
+                        double *yT, double *dJdy0, double *dJdp, int32_t *n_acc,
+                        int32_t *n_rej, int32_t *status, int nthreads)
+{
+    if (!ob || ob->k < 1 || !pb || !pb->controlled ||
+        pb->system == ORC_SYS_LOTKA_VOLTERRA_SHARED)
+        return -1;
+    return solve_batch_impl(pb, ob, B, b_begin, b_count, y0, p, NULL, 0, NULL, J, yT, dJdy0, dJdp,
+                            NULL, n_acc, n_rej, status, nthreads);
+}
+
+static int solve_batch_impl(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
+                            int64_t b_count, const double *y0, const double *p,
+                            const double *p_shared, int cost, const double *cost_w, double *J,
+                            double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
+                            int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads)
+{
     if (!pb || pb->n <= 0 || pb->n > ORC_MAX_N || b_begin < 0 || b_begin + b_count > B) return -1;
-    if (cost == ORC_COST_LINEAR && !cost_w) return -1;
+    if (!ob && cost == ORC_COST_LINEAR && !cost_w) return -1;
     int shared = pb->system == ORC_SYS_LOTKA_VOLTERRA_SHARED;
     if (shared && !p_shared) return -1;
     batch_ctx c;
@@ -824,6 +951,7 @@ int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b
     c.status = status;
     c.next = b_begin;
     c.np_full = shared ? pb->n + pb->n * pb->n : pb->np;
+    c.ob = ob;
     pthread_mutex_init(&c.mtx, NULL);
     double *acc = NULL;
     if (shared) {

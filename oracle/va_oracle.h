@@ -135,6 +135,16 @@ typedef struct orc_stats {
     int32_t status;
 } orc_stats;
 
+/* Observation schedule shared by a batch: k times t[0..k-1], increasing in the
+ * direction of integration and inside [t0, t1]; values y[k][N].  The running
+ * cost is J = sum_k sum_i (y_i(t_k) - y[k][i])^2 (no reference counterpart: the
+ * reference's cost is terminal only). */
+typedef struct orc_obs {
+    int k;
+    const double *t;
+    const double *y;
+} orc_obs;
+
 /* Tableau lookup; returns 0 on success. */
 int orc_get_tableau(int id, orc_tableau *out);
 
@@ -162,6 +172,17 @@ void orc_traj_free(orc_trajectory *tr);
  * (t, h, err, accepted) to log[4*k..] up to log_cap attempts. */
 int64_t orc_forward(const orc_problem *pb, double *y, const double *p, orc_trajectory *tr,
                     orc_stats *st, double *log, int64_t log_cap, int64_t *log_len);
+
+/* Forward solve with observation stops (controlled loop only).  obs_row[k]
+ * receives the stored-point index of observation k; *J the running cost. */
+int64_t orc_forward_obs(const orc_problem *pb, const orc_obs *ob, double *y, const double *p,
+                        orc_trajectory *tr, orc_stats *st, int32_t *obs_row, double *J);
+
+/* Reverse sweep of the observation cost: lambda (in: terminal seed, usually 0;
+ * out: dJ/dy0) jumps by 2 (y_n - y_obs_k) at every observation point. */
+void orc_backward_obs(const orc_problem *pb, const orc_obs *ob, const double *p,
+                      const orc_trajectory *tr, const int32_t *obs_row, double *lambda,
+                      double *mu);
 
 /* Reverse sweep over a stored trajectory for one seed.
  * lambda: in dJ/dy(T), out dJ/dy0 (overwritten).  mu: += dJ/dp. */
@@ -192,6 +213,13 @@ int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b
                     int cost, const double *cost_w,
                     double *J, double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
                     int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads);
+
+/* Batched solve of the observation cost (controlled loop): J[B] = running cost,
+ * dJdy0 / dJdp its gradient; the observations are shared by the batch. */
+int orc_solve_batch_obs(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
+                        int64_t b_count, const double *y0, const double *p, double *J,
+                        double *yT, double *dJdy0, double *dJdp, int32_t *n_acc,
+                        int32_t *n_rej, int32_t *status, int nthreads);
 
 #ifdef __cplusplus
 }

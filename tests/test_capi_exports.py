@@ -27,9 +27,10 @@ def test_library_exports_every_declared_symbol():
 
 def test_problem_struct_layout_matches_header():
     import vectorizedadjoint_amd as va
-    # int32 x6, int64, double x6, int32 x6, int64 x2 -> 120 bytes
-    assert ctypes.sizeof(va.VaProblem) == 120
+    # int32 x6, int64, double x6, int32 x6, int64 x2, int32 x2 -> 128 bytes
+    assert ctypes.sizeof(va.VaProblem) == 128
     assert va.VaProblem.batch.offset == 24 and va.VaProblem.max_attempts.offset == 80
     assert va.VaProblem.instances_per_wave.offset == 96
     assert va.VaProblem.lanes.offset == 104
     assert va.VaProblem.trajectory_budget_bytes.offset == 112
+    assert va.VaProblem.n_obs.offset == 120 and va.VaProblem.checkpoint_slots.offset == 100

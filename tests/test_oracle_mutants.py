@@ -12,6 +12,7 @@ import oracle as O
 import test_oracle_adjoint as TA
 import test_oracle_forward as TF
 import test_oracle_math as TM
+import test_oracle_observations as TO
 
 HERE = os.path.dirname(os.path.abspath(O.__file__))
 
@@ -54,6 +55,8 @@ MUTANTS = {
         lambda: TA.test_adjoint_matches_complex_step_of_frozen_map(_vdp_case())]),
     4: ("off-by-one stage index in the stage sums", [
         lambda: TF.test_oscillator_against_closed_form(O.TAB_DOPRI5, 1e-6)]),
+    6: ("wrong sign of the observation jump", [
+        lambda: TO.test_observation_adjoint_matches_complex_step(O.TAB_DOPRI5)]),
     5: ("wrong step-shrink exponent", [
         lambda: TF.test_controller_invariants(O.TAB_DOPRI5, 5, 4)]),
 }

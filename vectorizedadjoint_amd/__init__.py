@@ -25,11 +25,12 @@ from . import constants as _K
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvectorizedadjoint_amd.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
-    "va_sensitivity_matrix", "va_solve", "va_trajectory", "va_max_steps_seen", "va_last_error",
+    "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_trajectory",
+    "va_max_steps_seen", "va_last_error",
 )
 
 
@@ -42,8 +43,8 @@ class VaProblem(C.Structure):
         ("abs_tol", C.c_double), ("rel_tol", C.c_double), ("system_constant", C.c_double),
         ("max_attempts", C.c_int32), ("max_steps", C.c_int32), ("mode", C.c_int32),
         ("block_size", C.c_int32), ("instances_per_wave", C.c_int32),
-        ("checkpoint_interval", C.c_int32), ("lanes", C.c_int64),
-        ("trajectory_budget_bytes", C.c_int64),
+        ("checkpoint_slots", C.c_int32), ("lanes", C.c_int64),
+        ("trajectory_budget_bytes", C.c_int64), ("n_obs", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -79,11 +80,12 @@ def lib() -> C.CDLL:
         L.va_sensitivity_matrix.argtypes = [vp, vp, vp, vp]
         L.va_solve.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.va_max_steps_seen.argtypes = [vp, C.POINTER(C.c_int32), vp]
+        L.va_solve_observed.argtypes = [vp] * 13
         L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
         L.va_last_error.argtypes = []
         L.va_last_error.restype = C.c_char_p
         for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
-                  "va_trajectory", "va_max_steps_seen"):
+                  "va_solve_observed", "va_trajectory", "va_max_steps_seen"):
             getattr(L, f).restype = C.c_int
         _lib_handle = L
     return _lib_handle
@@ -115,6 +117,8 @@ class Problem:
     instances_per_wave: int = 0
     lanes: int = 0
     trajectory_budget_bytes: int = 0
+    n_obs: int = 0
+    checkpoint_slots: int = 0
 
     def to_c(self) -> VaProblem:
         pb = VaProblem()
@@ -130,6 +134,7 @@ class Problem:
         pb.mode, pb.block_size, pb.lanes = self.mode, self.block_size, self.lanes
         pb.instances_per_wave = self.instances_per_wave
         pb.trajectory_budget_bytes = self.trajectory_budget_bytes
+        pb.n_obs, pb.checkpoint_slots = self.n_obs, self.checkpoint_slots
         return pb
 
     @classmethod
@@ -247,6 +252,35 @@ class Solver:
                               _ptr(out["J"]), _ptr(out["yT"]), _ptr(out["dJdy0"]),
                               _ptr(out["dJdp"]), _ptr(out.get("n_acc")), _ptr(out.get("n_rej")),
                               _ptr(out.get("status")), _stream(stream)))
+        self.generation += 1
+        return out
+
+    def solve_observed(self, y0, p, obs_t, obs_y, out=None, stream=None):
+        """Forward with stops at obs_t [K], running cost J = sum_k |y(t_k) - obs_y[k]|^2
+        (obs_y [K, N], shared by the batch) and its gradient.  Returns a dict of device
+        tensors J, yT, dJdy0, dJdp, n_acc, n_rej, status."""
+        import torch
+        N, P, B = self.problem.n_state, self.problem.n_param, self.problem.batch
+        K = self.problem.n_obs
+        dev = y0.device
+        if out is None:
+            out = {
+                "J": torch.empty(B, dtype=torch.float64, device=dev),
+                "yT": torch.empty((N, B), dtype=torch.float64, device=dev),
+                "dJdy0": torch.empty((N, B), dtype=torch.float64, device=dev),
+                "dJdp": torch.empty((P, B), dtype=torch.float64, device=dev),
+                "n_acc": torch.empty(B, dtype=torch.int32, device=dev),
+                "n_rej": torch.empty(B, dtype=torch.int32, device=dev),
+                "status": torch.empty(B, dtype=torch.int32, device=dev),
+            }
+        self._shape_check(y0, (N, B), "f64")
+        self._shape_check(p, (P, B), "f64")
+        self._shape_check(obs_t, (K,), "f64")
+        self._shape_check(obs_y, (K, N), "f64")
+        _check(lib().va_solve_observed(
+            self._h, _ptr(y0), _ptr(p), _ptr(obs_t), _ptr(obs_y), _ptr(out["J"]),
+            _ptr(out["yT"]), _ptr(out["dJdy0"]), _ptr(out["dJdp"]), _ptr(out.get("n_acc")),
+            _ptr(out.get("n_rej")), _ptr(out.get("status")), _stream(stream)))
         self.generation += 1
         return out
 

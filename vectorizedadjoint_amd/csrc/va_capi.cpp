@@ -87,8 +87,8 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
 
 // Longest-first order of the reverse sweep.  The reverse of one instance is a
 // chain of n_acc dependent steps; when the long chains start last they set the
-// end of the launch.  Instances are bucketed by accepted-step count
-// (VA_ORDER_BUCKETS buckets of 16 steps, the last open-ended) and the reverse
+// end of the launch.  Instances are bucketed by accepted-step count (buckets of
+// 16 steps below 512, of 256 steps above, the last open-ended) and the reverse
 // takes them bucket by bucket from the longest; within a bucket the order is
 // arbitrary.  Each instance's result does not depend on the order.
 constexpr int VA_ORDER_BUCKETS = 64;
@@ -96,7 +96,7 @@ constexpr int VA_ORDER_BUCKETS = 64;
 __device__ __forceinline__ int order_bucket(int32_t n_acc, int32_t status)
 {
     if (status != 0) return 0;  // failed instances only write NaN: last
-    const int k = n_acc >> 4;
+    const int k = n_acc < 512 ? (n_acc >> 4) : 32 + ((n_acc - 512) >> 8);
     return k < VA_ORDER_BUCKETS - 1 ? k : VA_ORDER_BUCKETS - 1;
 }
 
@@ -167,6 +167,11 @@ struct va_handle {
     int32_t *counters = nullptr;  // [0] fwd queue, [1] rev queue, [2] max steps, [3] active
     int32_t *order = nullptr;     // [B] reverse order, longest first
     int32_t *order_tmp = nullptr; // [2 * VA_ORDER_BUCKETS] histogram, cursors
+    // observed solves
+    int64_t lanes_obs_fwd = 0, lanes_obs_rev = 0;
+    int32_t obs_ckpt = 0, obs_slots = 0, obs_stride_max = 0;
+    double *obs_ck = nullptr, *obs_seg = nullptr;
+    int32_t *obs_stride = nullptr, *obs_row = nullptr;
     int32_t *host_counter = nullptr;
     bool have_forward = false;
 };
@@ -213,6 +218,16 @@ va_status check_handle(const va_handle *h)
     return VA_OK;
 }
 
+// Handles that keep checkpoints have no per-step store: only va_solve_observed.
+va_status check_rows(const va_handle *h)
+{
+    if (h->obs_ckpt)
+        return fail(VA_ERR_STATE,
+                    "this handle keeps checkpoints instead of a trajectory store: use "
+                    "va_solve_observed");
+    return VA_OK;
+}
+
 void release(va_handle *h)
 {
     if (!h) return;
@@ -229,7 +244,28 @@ void release(va_handle *h)
     hipFree(h->counters);
     hipFree(h->order);
     hipFree(h->order_tmp);
+    hipFree(h->obs_ck);
+    hipFree(h->obs_seg);
+    hipFree(h->obs_stride);
+    hipFree(h->obs_row);
     if (h->host_counter) hipHostFree(h->host_counter);
+}
+
+// Longest-first order over the instances of the last forward (into h->order).
+va_status order_longest_first(va_handle *h, hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st = fill_i32(h->order_tmp, 0, 2 * VA_ORDER_BUCKETS, s);
+    if (st) return st;
+    const int gb = blocks_for(B, 256);
+    hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
+                       h->order_tmp);
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
+                       h->order_tmp + VA_ORDER_BUCKETS);
+    hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
+                       h->order_tmp + VA_ORDER_BUCKETS, h->order);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
 }
 
 // ---------------- persistent form ----------------
@@ -260,16 +296,7 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->counters + 1, 0, 1, s);
     if (st) return st;
-    // longest-first order over the instances of the last forward
-    if ((st = fill_i32(h->order_tmp, 0, 2 * VA_ORDER_BUCKETS, s))) return st;
-    const int gb = blocks_for(B, 256);
-    hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
-                       h->order_tmp);
-    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
-                       h->order_tmp + VA_ORDER_BUCKETS);
-    hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
-                       h->order_tmp + VA_ORDER_BUCKETS, h->order);
-    VA_HIP(hipGetLastError());
+    if ((st = order_longest_first(h, s))) return st;
     va::RevArgs a;
     a.sp = solve_params(h);
     a.order = h->order;
@@ -425,9 +452,13 @@ va_status va_create(const va_problem *pb, va_handle **out)
         (pb->tableau == VA_TABLEAU_EULER || pb->tableau == VA_TABLEAU_RK4))
         return fail(VA_ERR_INVALID,
                     "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
-    va::KernelSet ks;
-    if (pb->checkpoint_interval != 0)
-        return fail(VA_ERR_INVALID, "checkpoint_interval is reserved and must be 0");
+    va::KernelSet ks{};
+    if (pb->reserved != 0) return fail(VA_ERR_INVALID, "va_problem.reserved must be 0");
+    if (pb->n_obs < 0) return fail(VA_ERR_INVALID, "n_obs must be >= 0");
+    if (pb->checkpoint_slots < 0 || pb->checkpoint_slots == 1 || pb->checkpoint_slots % 2 != 0)
+        return fail(VA_ERR_INVALID, "checkpoint_slots must be 0 or an even number >= 2");
+    if (pb->checkpoint_slots > 0 && pb->n_obs == 0)
+        return fail(VA_ERR_INVALID, "checkpoint_slots applies to observed solves (n_obs > 0)");
     if (pb->instances_per_wave < 0 || pb->instances_per_wave > 64)
         return fail(VA_ERR_INVALID, "instances_per_wave must be in [0, 64]");
     if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive,
@@ -439,6 +470,10 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (pb->n_param != ks.np)
         return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
                     pb->system, pb->n_state, ks.np, pb->n_param);
+    if (pb->n_obs > 0 && (!ks.obs_fwd || pb->mode != VA_MODE_PERSISTENT))
+        return fail(VA_ERR_UNSUPPORTED,
+                    "observed solves need an adaptive error stepper on a one-lane system "
+                    "(Van der Pol, oscillators) in the persistent mode");
 
     va_handle *h = new (std::nothrow) va_handle();
     if (!h) return fail(VA_ERR_OUT_OF_MEMORY, "host allocation");
@@ -465,14 +500,56 @@ va_status va_create(const va_problem *pb, va_handle **out)
         h->lanes_fwd = (int64_t)std::max(1, ks.occ_fwd(h->block)) * h->cu_count * per_block;
         h->lanes_rev = (int64_t)std::max(1, ks.occ_rev(h->block)) * h->cu_count * per_block;
     }
-    {
+    const double budget = (double)pb->trajectory_budget_bytes;
+    bool full_store = true;  // per-step rows in h->traj / h->ptraj
+    if (pb->n_obs > 0) {
+        // observation handles: full rows while they fit the budget, else checkpoints
+        const int64_t per_block = h->block;
+        h->lanes_obs_fwd = (int64_t)std::max(1, ks.occ_obs_fwd(h->block)) * h->cu_count * per_block;
+        h->lanes_obs_rev = (int64_t)std::max(1, ks.occ_obs_rev(h->block)) * h->cu_count * per_block;
+        if (pb->lanes > 0) h->lanes_obs_fwd = h->lanes_obs_rev = pb->lanes;
+        const double fixed = 4.0 * (double)B * ((double)pb->n_obs + 1.0);
+        const double full = 8.0 * (double)B * ((double)pb->max_steps + 1.0) * (ks.n + 2) + fixed;
+        if (pb->checkpoint_slots == 0 && (budget <= 0.0 || full <= budget)) {
+            h->obs_ckpt = 0;
+        } else {
+            full_store = false;
+            h->obs_ckpt = 1;
+            const int64_t rev_threads =
+                (int64_t)blocks_for(std::min<int64_t>(h->lanes_obs_rev, B), h->block) * h->block;
+            auto cost = [&](int32_t slots, int32_t *stride_max) {
+                int32_t sm = 1;
+                while ((int64_t)sm * slots < (int64_t)pb->max_steps) sm *= 2;
+                *stride_max = sm;
+                return 8.0 * (double)B * slots * (ks.n + 3) +
+                       8.0 * (double)rev_threads * sm * (ks.n + 1) + fixed;
+            };
+            int32_t slots = pb->checkpoint_slots, sm = 0;
+            if (slots > 0) {
+                cost(slots, &sm);
+            } else {
+                // the most slots that fit (shortest segments, smallest recompute buffers)
+                for (slots = 1 << 16; slots >= 2; slots /= 2)
+                    if (cost(slots, &sm) <= budget) break;
+                if (slots < 2) {
+                    delete h;
+                    return fail(VA_ERR_OUT_OF_MEMORY,
+                                "trajectory_budget_bytes %lld cannot hold checkpoints for %lld "
+                                "instances",
+                                (long long)pb->trajectory_budget_bytes, (long long)B);
+                }
+            }
+            h->obs_slots = slots;
+            h->obs_stride_max = sm;
+        }
+    } else {
         const size_t rows_b = (size_t)pb->max_steps + 1;
         const size_t per_inst =
             pb->mode == VA_MODE_PERSISTENT
                 ? rows_b * (ks.n + 2)
                 : rows_b * (2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n);
         const double store = (double)sizeof(double) * (double)per_inst * (double)B;
-        if (pb->trajectory_budget_bytes > 0 && store > (double)pb->trajectory_budget_bytes) {
+        if (budget > 0.0 && store > budget) {
             delete h;
             return fail(VA_ERR_OUT_OF_MEMORY,
                         "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
@@ -491,8 +568,19 @@ va_status va_create(const va_problem *pb, va_handle **out)
     alloc((void **)&h->counters, sizeof(int32_t) * 4);
     alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
+    if (pb->n_obs > 0) {
+        alloc((void **)&h->obs_row, sizeof(int32_t) * (size_t)B * pb->n_obs);
+        if (h->obs_ckpt) {
+            const int64_t rev_threads =
+                (int64_t)blocks_for(std::min<int64_t>(h->lanes_obs_rev, B), h->block) * h->block;
+            alloc((void **)&h->obs_ck, sizeof(double) * (size_t)B * h->obs_slots * (ks.n + 3));
+            alloc((void **)&h->obs_stride, sizeof(int32_t) * (size_t)B);
+            alloc((void **)&h->obs_seg,
+                  sizeof(double) * (size_t)rev_threads * h->obs_stride_max * (ks.n + 1));
+        }
+    }
     if (pb->mode == VA_MODE_PERSISTENT) {
-        alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
+        if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {
         const size_t F = 2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n;
         alloc((void **)&h->ptraj, sizeof(double) * rows * F * (size_t)B);
@@ -527,6 +615,7 @@ va_status va_forward(va_handle *h, double *y, const double *p, int32_t *n_accept
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if ((st = check_rows(h))) return st;
     if (!y || !p) return fail(VA_ERR_INVALID, "y and p are required");
     hipStream_t s = (hipStream_t)stream;
     st = h->pb.mode == VA_MODE_PLAIN ? forward_plain(h, y, p, s) : forward_persistent(h, y, p, s);
@@ -540,6 +629,7 @@ va_status va_adjoint(va_handle *h, const double *p, int32_t nout, double *lambda
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if ((st = check_rows(h))) return st;
     if (!h->have_forward) return fail(VA_ERR_STATE, "va_adjoint needs a va_forward first");
     if (!p || !lambda || !mu || nout < 1)
         return fail(VA_ERR_INVALID, "p, lambda, mu required and nout >= 1");
@@ -562,6 +652,7 @@ va_status va_sensitivity_matrix(va_handle *h, const double *p, double *S, void *
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if ((st = check_rows(h))) return st;
     if (!p || !S) return fail(VA_ERR_INVALID, "p and S are required");
     hipStream_t s = (hipStream_t)stream;
     const int64_t B = h->pb.batch;
@@ -586,6 +677,7 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if ((st = check_rows(h))) return st;
     if (!y0 || !p || !J || !yT || !dJdy0 || !dJdp)
         return fail(VA_ERR_INVALID, "y0, p, J, yT, dJdy0, dJdp are required");
     if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
@@ -618,11 +710,78 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
     return copy_out(h, n_accepted, n_rejected, lane_status, s);
 }
 
+va_status va_solve_observed(va_handle *h, const double *y0, const double *p, const double *obs_t,
+                            const double *obs_y, double *J, double *yT, double *dJdy0,
+                            double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                            int32_t *lane_status, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (h->pb.n_obs <= 0)
+        return fail(VA_ERR_STATE, "va_solve_observed needs a handle created with n_obs > 0");
+    if (!y0 || !p || !obs_t || !obs_y || !J || !yT || !dJdy0 || !dJdp)
+        return fail(VA_ERR_INVALID, "y0, p, obs_t, obs_y, J, yT, dJdy0, dJdp are required");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = h->pb.batch;
+    const int n = h->ks.n;
+    if (yT != y0) {
+        hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)n * B, 256)), dim3(256), 0, s, yT,
+                           y0, (int64_t)n * B);
+        VA_HIP(hipGetLastError());
+    }
+    if ((st = fill_i32(h->counters, 0, 4, s))) return st;
+    va::ObsStore os;
+    os.ckpt = h->obs_ckpt;
+    os.traj = h->traj;
+    os.ck = h->obs_ck;
+    os.slots = h->obs_slots;
+    os.stride_max = h->obs_stride_max;
+    os.stride = h->obs_stride;
+    os.seg = h->obs_seg;
+    os.obs_row = h->obs_row;
+    va::ObsFwdArgs f;
+    f.sp = solve_params(h);
+    f.y = yT;
+    f.p = p;
+    f.n_acc = h->n_acc;
+    f.n_rej = h->n_rej;
+    f.status = h->status;
+    f.counter = h->counters + 0;
+    f.max_steps_seen = h->counters + 2;
+    f.obs_t = obs_t;
+    f.obs_y = obs_y;
+    f.K = h->pb.n_obs;
+    f.J = J;
+    f.st = os;
+    VA_HIP(h->ks.obs_fwd(f, blocks_for(std::min<int64_t>(h->lanes_obs_fwd, B), h->block),
+                         h->block, s));
+    h->have_forward = !h->obs_ckpt;
+    if ((st = order_longest_first(h, s))) return st;
+    va::ObsRevArgs r;
+    r.sp = f.sp;
+    r.p = p;
+    r.yT = yT;
+    r.n_acc = h->n_acc;
+    r.status = h->status;
+    r.obs_y = obs_y;
+    r.obs_t = obs_t;
+    r.K = h->pb.n_obs;
+    r.dJdy0 = dJdy0;
+    r.dJdp = dJdp;
+    r.counter = h->counters + 1;
+    r.order = h->order;
+    r.st = os;
+    VA_HIP(h->ks.obs_rev(r, blocks_for(std::min<int64_t>(h->lanes_obs_rev, B), h->block),
+                         h->block, s));
+    return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
 va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
                         int64_t *n_rows, void *stream)
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if ((st = check_rows(h))) return st;
     if (!h->have_forward) return fail(VA_ERR_STATE, "va_trajectory needs a va_forward first");
     if (!n_rows || instance < 0 || instance >= h->pb.batch || max_rows < 0 ||
         (max_rows > 0 && !rows))

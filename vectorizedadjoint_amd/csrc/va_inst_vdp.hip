@@ -1,11 +1,13 @@
 // va_inst_vdp.hip -- kernel instantiations for the Van der Pol system.
-#include "va_kernels_impl.hpp"
+#include "va_observed.hpp"
 
 namespace va {
 
 bool kernels_van_der_pol(int tableau, int adaptive, KernelSet *out)
 {
-    return pick_tableau<VanDerPol>(tableau, adaptive, out);
+    if (!pick_tableau<VanDerPol>(tableau, adaptive, out)) return false;
+    pick_observed<VanDerPol>(tableau, adaptive, out);
+    return true;
 }
 
 }  // namespace va

@@ -76,6 +76,48 @@ struct PlainRevArgs {
     int32_t k;             // reverse launch index
 };
 
+// Observation cost (forced stops at obs_t, running cost, lambda jumps) with the
+// trajectory either stored in full or kept as checkpoints.
+struct ObsStore {
+    int32_t ckpt;          // 0: full rows (t, h, y) in traj; 1: checkpoints in ck
+    double *traj;          // [B][cap+1][N+2]
+    double *ck;            // [B][slots][N+3] (t, dt to try, observations taken, y)
+    int32_t slots;         // checkpoints per instance (even)
+    int32_t stride_max;    // largest checkpoint spacing; slots * stride_max >= cap
+    int32_t *stride;       // [B] spacing each instance ended with
+    double *seg;           // [lanes][stride_max][N+1] (h, y) rows recomputed by the reverse
+    int32_t *obs_row;      // [B][K] stored-point index of each observation
+};
+
+struct ObsFwdArgs {
+    SolveParams sp;
+    double *y;             // [N][B] in y0, out y(T)
+    const double *p;
+    int32_t *n_acc, *n_rej, *status;
+    int32_t *counter;
+    int32_t *max_steps_seen;
+    const double *obs_t;   // [K]
+    const double *obs_y;   // [K][N]
+    int32_t K;
+    double *J;             // [B]
+    ObsStore st;
+};
+
+struct ObsRevArgs {
+    SolveParams sp;
+    const double *p;
+    const double *yT;      // [N][B]
+    const int32_t *n_acc, *status;
+    const double *obs_y;   // [K][N]
+    const double *obs_t;
+    int32_t K;
+    double *dJdy0;         // [N][B]
+    double *dJdp;          // [P][B]
+    int32_t *counter;
+    const int32_t *order;
+    ObsStore st;
+};
+
 struct KernelSet {
     int n, np, s_rev, s_fwd;
     int flop_rhs, flop_vjp;
@@ -87,6 +129,11 @@ struct KernelSet {
     int (*occ_rev)(int block);
     int (*slots)(int block);     // instances one block holds at a time
     int layout;                  // 0: one lane per instance, 1: one group of N lanes per instance
+    // observation-cost kernels (null where the system/tableau has none)
+    hipError_t (*obs_fwd)(const ObsFwdArgs &, int grid, int block, hipStream_t);
+    hipError_t (*obs_rev)(const ObsRevArgs &, int grid, int block, hipStream_t);
+    int (*occ_obs_fwd)(int block);
+    int (*occ_obs_rev)(int block);
 };
 
 // Returns false when (system, n, tableau, adaptive) has no compiled kernels.

@@ -1,0 +1,392 @@
+// va_observed.hpp -- observation-cost forward and reverse sweeps (one lane per
+// instance), with the trajectory stored in full or kept as checkpoints.
+//
+// No reference counterpart: the reference's cost is terminal only
+// (examples/VanDerPol/main.cpp:98-117).  The forward is the controlled loop of
+// lib/include/detail/runge_kutta.hpp:74-118 run segment by segment: a step that
+// would pass the next observation time is shortened to land on it, as at t1
+// (:98-101); after such a shortened step is accepted, the next step size is the
+// larger of the controller's proposal and the size before shortening (what
+// Boost.odeint 1.74 integrate_times does for controlled steppers).  At each
+// observation time the running cost J += |y - y_obs_k|^2 is accumulated and the
+// stored-point index recorded.  The reverse is the discrete adjoint of
+// detail/backpropagation.hpp:83-158 with lambda += 2 (y_n - y_obs_k) at every
+// observation point.
+//
+// Checkpoints.  With a trajectory budget too small for every row, an instance
+// keeps `slots` checkpoints (t, dt to try, observations taken, y) at every
+// `stride`-th accepted step; when the slots fill up, every other checkpoint is
+// dropped and the stride doubles, so any step count up to slots * stride_max
+// fits.  The reverse rebuilds each segment's rows from its checkpoint by
+// replaying the controlled loop (the same decisions, so bit-identical rows, and
+// f(y) from the checkpoint equals the FSAL value carried by the forward), then
+// sweeps the segment backwards.  Memory: slots rows per instance plus
+// stride_max rows per lane; work: one extra forward pass.
+#pragma once
+
+#include "va_kernels_impl.hpp"
+
+namespace va {
+
+// Observation jumps at stored point `point` (state yn), newest observation first.
+template <int N>
+__device__ __forceinline__ int obs_jumps(const int32_t *orow, const double *obs_y, int kobs,
+                                         int32_t point, const double *yn, double *lam)
+{
+    while (kobs >= 0 && orow[kobs] == point) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lam[i] = fma(2.0, yn[i] - obs_y[kobs * N + i], lam[i]);
+        --kobs;
+    }
+    return kobs;
+}
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+template <class Sys, class Tab, bool Ckpt>
+__global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
+{
+    constexpr int N = Sys::N;
+    constexpr int F = N + 2, FC = N + 3;
+    using R = RK<Sys, Tab>;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int K = a.K;
+
+    typename Sys::Params q;
+    double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
+    double t = 0.0, dt = 0.0, dt_keep = 0.0, Jacc = 0.0;
+    int64_t b = 0;
+    int32_t count = 0, rej = 0, tries = 0, st = LANE_OK, kobs = 0, stride = 1, nck = 0;
+    bool new_step = true, have_k0 = false, need = true, clipped = false;
+    double *rows = nullptr, *ck = nullptr;
+    int32_t *orow = nullptr;
+
+    for (;;) {
+        if (need) {
+            b = atomicAdd(a.counter, 1);
+            if (b >= B) break;
+            Sys::load(q, a.p, B, b, sp.kconst);
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = a.y[(int64_t)i * B + b];
+            t = sp.t0;
+            dt = sp.dt0;
+            Jacc = 0.0;
+            count = rej = tries = kobs = nck = 0;
+            stride = 1;
+            st = LANE_OK;
+            new_step = true;
+            have_k0 = false;
+            need = false;
+            if constexpr (Ckpt)
+                ck = a.st.ck + (size_t)b * (size_t)a.st.slots * FC;
+            else
+                rows = a.st.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+            orow = a.st.obs_row + (size_t)b * (size_t)K;
+        }
+        bool finish = false;
+        if (new_step) {
+            // observations reached at the current point, then the next target
+            double target = sp.t1;
+            bool is_obs = false;
+            for (;;) {
+                is_obs = kobs < K;
+                target = is_obs ? a.obs_t[kobs] : sp.t1;
+                if (!is_obs || less_with_sign(t, target, dt)) break;
+                orow[kobs] = count;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    const double d = y[i] - a.obs_y[kobs * N + i];
+                    Jacc = fma(d, d, Jacc);
+                }
+                ++kobs;
+            }
+            if (!less_with_sign(t, target, dt)) {
+                finish = true;
+            } else if (count >= sp.cap) {
+                st = LANE_MAX_STEPS;
+                finish = true;
+            } else {
+                if constexpr (Ckpt) {
+                    if (count % stride == 0) {
+                        if (nck >= a.st.slots) {
+                            // drop every other checkpoint, double the spacing
+                            for (int m = 1; m < a.st.slots / 2; ++m)
+#pragma unroll
+                                for (int f = 0; f < FC; ++f)
+                                    ck[(size_t)m * FC + f] = ck[(size_t)(2 * m) * FC + f];
+                            stride *= 2;
+                            nck = a.st.slots / 2;
+                        }
+                        double *c = ck + (size_t)nck * FC;
+                        c[0] = t;
+                        c[1] = dt;
+                        c[2] = (double)kobs;
+#pragma unroll
+                        for (int i = 0; i < N; ++i) c[3 + i] = y[i];
+                        ++nck;
+                    }
+                } else {
+                    double *r = rows + (size_t)count * F;
+                    r[0] = t;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) r[2 + i] = y[i];
+                }
+                clipped = false;
+                dt_keep = dt;
+                if (less_with_sign(target, t + dt, dt)) {
+                    dt = target - t;
+                    clipped = is_obs;
+                }
+                if (!Tab::FSAL || !have_k0) {
+                    Sys::rhs(q, y, Kst[0]);
+                    have_k0 = true;
+                }
+                tries = 0;
+                new_step = false;
+            }
+        }
+        if (!finish) {
+            const double h = dt;
+            R::attempt(q, y, Kst, h, ynew, xerr);
+            const double err = R::error_norm(sp.atol, sp.rtol, h, y, Kst[0], xerr);
+            double dt_next;
+            if (!control_step<Tab>(dt, err, dt_next)) {
+                ++rej;
+                clipped = false;
+            } else {
+                if constexpr (!Ckpt) rows[(size_t)count * F + 1] = dt;
+                t = t + dt;
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = ynew[i];
+                if constexpr (Tab::FSAL) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) Kst[0][i] = Kst[Tab::SF - 1][i];
+                }
+                if (clipped && fabs(dt_keep) > fabs(dt_next)) dt_next = dt_keep;
+                ++count;
+                new_step = true;
+            }
+            dt = dt_next;
+            ++tries;
+            if (tries > sp.max_attempts) {
+                st = LANE_MAX_FAIL;
+                finish = true;
+            }
+        }
+        if (finish) {
+            if (st == LANE_OK) {
+                if constexpr (!Ckpt) {
+                    double *r = rows + (size_t)count * F;
+                    r[0] = t;
+                    r[1] = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) r[2 + i] = y[i];
+                }
+                if (!all_finite<N>(y)) st = LANE_NONFINITE;
+            }
+#pragma unroll
+            for (int i = 0; i < N; ++i) a.y[(int64_t)i * B + b] = y[i];
+            a.J[b] = st == LANE_OK ? Jacc : __builtin_nan("");
+            a.n_acc[b] = count;
+            a.n_rej[b] = rej;
+            a.status[b] = st;
+            if constexpr (Ckpt) a.st.stride[b] = stride;
+            atomicMax(a.max_steps_seen, count);
+            need = true;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse
+// ---------------------------------------------------------------------------
+
+// Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg) from the checkpoint c, replaying
+// the forward's controlled loop with the same observation stops.
+template <class Sys, class Tab>
+__device__ void recompute_segment(const typename Sys::Params &q, const SolveParams &sp,
+                                  const double *obs_t, int K, const double *c, int32_t len,
+                                  double *seg)
+{
+    constexpr int N = Sys::N, FS = N + 1;
+    using R = RK<Sys, Tab>;
+    double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
+    double t = c[0], dt = c[1];
+    int kobs = (int)c[2];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y[i] = c[3 + i];
+    // f(y) here equals the FSAL value the forward carried into this step: the
+    // last stage input of DOPRI5 is y_{n+1} computed with the same operations.
+    Sys::rhs(q, y, Kst[0]);
+    for (int32_t k = 0; k < len; ++k) {
+        double target;
+        bool is_obs;
+        for (;;) {
+            is_obs = kobs < K;
+            target = is_obs ? obs_t[kobs] : sp.t1;
+            if (!is_obs || less_with_sign(t, target, dt)) break;
+            ++kobs;
+        }
+        double *r = seg + (size_t)k * FS;
+#pragma unroll
+        for (int i = 0; i < N; ++i) r[1 + i] = y[i];
+        bool clipped = false;
+        const double dt_keep = dt;
+        if (less_with_sign(target, t + dt, dt)) {
+            dt = target - t;
+            clipped = is_obs;
+        }
+        if (!Tab::FSAL && k > 0) Sys::rhs(q, y, Kst[0]);
+        for (int32_t tries = 0; tries <= sp.max_attempts; ++tries) {
+            const double h = dt;
+            R::attempt(q, y, Kst, h, ynew, xerr);
+            const double err = R::error_norm(sp.atol, sp.rtol, h, y, Kst[0], xerr);
+            double dt_next;
+            const bool ok = control_step<Tab>(dt, err, dt_next);
+            if (ok) {
+                r[0] = dt;
+                t = t + dt;
+#pragma unroll
+                for (int i = 0; i < N; ++i) y[i] = ynew[i];
+                if constexpr (Tab::FSAL) {
+#pragma unroll
+                    for (int i = 0; i < N; ++i) Kst[0][i] = Kst[Tab::SF - 1][i];
+                }
+                if (clipped && fabs(dt_keep) > fabs(dt_next)) dt_next = dt_keep;
+                dt = dt_next;
+                break;
+            }
+            clipped = false;
+            dt = dt_next;
+        }
+    }
+}
+
+template <class Sys, class Tab, bool Ckpt>
+__global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
+{
+    constexpr int N = Sys::N, P = Sys::P;
+    constexpr int F = N + 2, FC = N + 3, FS = N + 1;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int K = a.K;
+    const double qnan = __builtin_nan("");
+    const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double *seg = nullptr;
+    if constexpr (Ckpt) seg = a.st.seg + (size_t)gtid * (size_t)a.st.stride_max * FS;
+
+    for (;;) {
+        int64_t b = atomicAdd(a.counter, 1);
+        if (b >= B) break;
+        if (a.order) b = a.order[b];
+        if (a.status[b] != LANE_OK) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) a.dJdy0[(int64_t)i * B + b] = qnan;
+            for (int k = 0; k < P; ++k) a.dJdp[(int64_t)k * B + b] = qnan;
+            continue;
+        }
+        typename Sys::Params q;
+        Sys::load(q, a.p, B, b, sp.kconst);
+        double lam[N], pbar[P], yT[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            lam[i] = 0.0;
+            yT[i] = a.yT[(int64_t)i * B + b];
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) pbar[k] = 0.0;
+        const int32_t cnt = a.n_acc[b];
+        const int32_t *orow = a.st.obs_row + (size_t)b * (size_t)K;
+        int kobs = obs_jumps<N>(orow, a.obs_y, K - 1, cnt, yT, lam);
+        if constexpr (!Ckpt) {
+            const double *rows = a.st.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+            for (int32_t n = cnt - 1; n >= 0; --n) {
+                const double *r = rows + (size_t)n * F;
+                back_prop_step<Sys, Tab>(q, r[1], r + 2, lam, pbar);
+                kobs = obs_jumps<N>(orow, a.obs_y, kobs, n, r + 2, lam);
+            }
+        } else {
+            const int32_t stride = a.st.stride[b];
+            const double *ck = a.st.ck + (size_t)b * (size_t)a.st.slots * FC;
+            const int32_t nseg = (cnt + stride - 1) / stride;
+            for (int32_t j = nseg - 1; j >= 0; --j) {
+                const int32_t n0 = j * stride;
+                const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
+                recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, ck + (size_t)j * FC, len, seg);
+                for (int32_t k = len - 1; k >= 0; --k) {
+                    const double *r = seg + (size_t)k * FS;
+                    back_prop_step<Sys, Tab>(q, r[0], r + 1, lam, pbar);
+                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, n0 + k, r + 1, lam);
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.dJdy0[(int64_t)i * B + b] = lam[i];
+        for (int k = 0; k < P; ++k) a.dJdp[(int64_t)k * B + b] = 0.0 + pbar[k];
+    }
+}
+
+template <class Sys, class Tab>
+struct LaunchObs {
+    static hipError_t fwd(const ObsFwdArgs &a, int grid, int block, hipStream_t s)
+    {
+        if (a.st.ckpt)
+            hipLaunchKernelGGL((k_obs_fwd<Sys, Tab, true>), dim3(grid), dim3(block), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_obs_fwd<Sys, Tab, false>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static hipError_t rev(const ObsRevArgs &a, int grid, int block, hipStream_t s)
+    {
+        if (a.st.ckpt)
+            hipLaunchKernelGGL((k_obs_rev<Sys, Tab, true>), dim3(grid), dim3(block), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_obs_rev<Sys, Tab, false>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    // the smaller occupancy of the two storage variants, so one sizing serves both
+    static int occ_fwd(int block)
+    {
+        int n0 = 0, n1 = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_obs_fwd<Sys, Tab, false>, block,
+                                                         0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_obs_fwd<Sys, Tab, true>, block,
+                                                         0) != hipSuccess)
+            return 1;
+        return n0 < n1 ? n0 : n1;
+    }
+    static int occ_rev(int block)
+    {
+        int n0 = 0, n1 = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n0, k_obs_rev<Sys, Tab, false>, block,
+                                                         0) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n1, k_obs_rev<Sys, Tab, true>, block,
+                                                         0) != hipSuccess)
+            return 1;
+        return n0 < n1 ? n0 : n1;
+    }
+    static void apply(KernelSet *k)
+    {
+        k->obs_fwd = &fwd;
+        k->obs_rev = &rev;
+        k->occ_obs_fwd = &occ_fwd;
+        k->occ_obs_rev = &occ_rev;
+    }
+};
+
+// Adds the observation kernels for the adaptive error steppers.
+template <class Sys>
+inline void pick_observed(int tableau, int adaptive, KernelSet *out)
+{
+    if (!adaptive) return;
+    switch (tableau) {
+    case TAB_CK54: LaunchObs<Sys, CashKarp54>::apply(out); break;
+    case TAB_DOPRI5: LaunchObs<Sys, Dopri5>::apply(out); break;
+    case TAB_F78: LaunchObs<Sys, Fehlberg78>::apply(out); break;
+    default: break;
+    }
+}
+
+}  // namespace va

@@ -99,6 +99,13 @@ WORKLOADS = {
         65536, K.COST_SUM,
         "generalized Lotka-Volterra N=10, r~U(0.1,1), A_ij~N(0,0.2^2), A_ii=-U(0.5,1.5), "
         "x0~U(0.1,1), t in [0,50], Cash-Karp 5(4) tol 1e-8, J=sum x_i(T), B=65536"),
+    "vdp_obs": Workload(
+        "vdp_obs", 5, K.SYS_VAN_DER_POL, 2, 1, K.TAB_DOPRI5, 0.0, 20.0, 1e-3, 1e-8, 262144,
+        0,
+        "Van der Pol with mu log-uniform on [0.1, 100], y0=(2,0)+N(0,0.01^2), t in [0,20], "
+        "DOPRI5 tol 1e-8, J = sum_k |y(t_k) - y_obs(t_k)|^2 at t_k = 20k/50 (k=1..50), "
+        "observations from a seeded ground-truth mu, B=262144, 4 GB trajectory budget",
+        extra={"n_obs": 50, "budget_bytes": 4 << 30}),
     "glv128_shared": Workload(
         "glv128_shared", 4, K.SYS_LOTKA_VOLTERRA_SHARED, 128, 128, K.TAB_CASH_KARP54, 0.0, 50.0,
         1e-3, 1e-8, 16384, K.COST_SUM,
@@ -115,6 +122,12 @@ def make_inputs(w: Workload | str, B: int | None = None) -> dict:
     B = w.batch if B is None else int(B)
     s = w.seed
     N = w.n
+    if w.system == K.SYS_VAN_DER_POL and w.name == "vdp_obs":
+        mu = np.exp(math.log(0.1) + (math.log(100.0) - math.log(0.1)) * uniform(s, 0, B, 1))
+        nz = normal(s, 1, B, 2) * 0.01
+        y0 = np.stack([2.0 + nz[:, 0], nz[:, 1]])
+        return {"y0": np.ascontiguousarray(y0), "p": np.ascontiguousarray(mu.T.copy()),
+                "p_shared": None}
     if w.system == K.SYS_VAN_DER_POL:
         mu = 0.5 + 4.5 * uniform(s, 0, B, 1)
         nz = normal(s, 1, B, 2) * 0.01
@@ -146,3 +159,15 @@ def make_inputs(w: Workload | str, B: int | None = None) -> dict:
         return {"y0": np.ascontiguousarray(x0.T), "p": np.ascontiguousarray(r.T),
                 "p_shared": np.ascontiguousarray(A.reshape(N * N))}
     raise ValueError(f"no generator for system {w.system}")
+
+
+def observation_schedule(w: Workload | str):
+    """(t_obs [K], mu_true, y0_true) of an observed workload: K uniform times in
+    (t0, t1] and a ground-truth parameter drawn from the workload's seed."""
+    if isinstance(w, str):
+        w = WORKLOADS[w]
+    K_obs = int(w.extra["n_obs"])
+    t_obs = w.t0 + (w.t1 - w.t0) * np.arange(1, K_obs + 1) / K_obs
+    u = uniform(w.seed, 7, 1, 1)[0, 0]
+    mu_true = math.exp(math.log(0.1) + (math.log(100.0) - math.log(0.1)) * u)
+    return t_obs, mu_true, np.array([2.0, 0.0])
