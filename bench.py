@@ -95,6 +95,87 @@ def cpu_baseline(w, inputs, target_s=10.0):
                       f"{nthreads} threads"}
 
 
+def bench_observed(a, torch, va, I):
+    """The observed Van der Pol workload: forward with stops at 50 observation times,
+    running cost, adjoint with lambda jumps, checkpointed under a 4 GB trajectory budget."""
+    w = I.WORKLOADS["vdp_obs"]
+    B = w.batch
+    t_obs, mu_true, y0_true = I.observation_schedule(w)
+    K_obs = t_obs.size
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    # observations: the ground-truth trajectory solved on the device with every row stored
+    log(f"[bench] observations from mu* = {mu_true:.4f}")
+    with va.Solver(va.Problem.from_workload(w, batch=1, n_obs=K_obs, max_steps=65536)) as g:
+        tr_out = g.solve_observed(dev(y0_true.reshape(2, 1)), dev([[mu_true]]), dev(t_obs),
+                                  dev(np.zeros((K_obs, 2))))
+        torch.cuda.synchronize()
+        rows = g.trajectory(0)
+    t_rows = rows[:, 0]
+    idx = [int(np.argmin(np.abs(t_rows - tk))) for tk in t_obs]
+    y_obs = rows[idx, 2:4].copy()
+    assert np.all(np.abs(t_rows[idx] - t_obs) <= 1e-12 * w.t1), "observation rows not found"
+    inp = I.make_inputs(w, B)
+    y0, p = dev(inp["y0"]), dev(inp["p"])
+    ot, oy = dev(t_obs), dev(y_obs)
+    budget = int(w.extra["budget_bytes"])
+    pb = va.Problem.from_workload(w, n_obs=K_obs, max_steps=16384,
+                                  trajectory_budget_bytes=budget)
+    log(f"[bench] observed solve B={B}, budget {budget / 2**30:.0f} GiB")
+    with va.Solver(pb) as s:
+        out = None
+        for _ in range(a.warmup):
+            out = s.solve_observed(y0, p, ot, oy, out=out)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            out = s.solve_observed(y0, p, ot, oy, out=out)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t) / a.steps
+        st = out["status"].cpu().numpy()
+        n_acc = out["n_acc"].cpu().numpy()
+        n_rej = out["n_rej"].cpu().numpy()
+    n_ok = int((st == 0).sum())
+    from vectorizedadjoint_amd import flops as FL
+    fl = (FL.forward_flops(w.system, w.n, w.tableau, n_acc, n_rej) * 2.0
+          + FL.reverse_flops(w.system, w.n, w.tableau, n_acc[st == 0]))
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline (oracle, observed)")
+        import oracle as O
+        nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+        opb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
+                        abs_tol=w.tol, rel_tol=w.tol, max_steps=16384)
+        k = min(B, 2048 * nthreads)
+        tc = time.perf_counter()
+        O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), 0,
+                      obs=(t_obs, y_obs), nthreads=nthreads)
+        dtc = time.perf_counter() - tc
+        cpu = {"value": k / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
+               "sample": f"first {k} of {B} instances (same inputs and observations), "
+                         f"{dtc:.1f} s wall, {nthreads} threads"}
+    res = {
+        "metric": "observed adjoint gradients/sec, Van der Pol log-uniform mu, B=262144, "
+                  "50 observations, FP64, 4 GB trajectory budget",
+        "value": n_ok / sec, "unit": "gradients/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 inputs; observations from a seeded ground truth)",
+        "config": {"workload": "vdp_obs", "batch": B, "tableau": "dopri5", "tol": w.tol,
+                   "t_span": [w.t0, w.t1], "n_obs": K_obs, "mu_range": [0.1, 100.0],
+                   "trajectory_budget_bytes": budget, "instances_ok": n_ok},
+        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU", "kernel": "solve",
+                     "achieved": fl / sec / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": fl / sec / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_solve": fl, "note": "forward counted twice (recompute)"},
+        "cpu_baseline": cpu,
+        "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
+                               "accepted_min": int(n_acc.min()),
+                               "accepted_max": int(n_acc.max())},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +183,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
+    ap.add_argument("--workload", default="glv10", choices=["glv10", "vdp_obs"],
+                    help="glv10: the headline metric; vdp_obs: the observed Van der Pol solve")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -125,6 +208,8 @@ def main(argv=None) -> int:
     import vectorizedadjoint_amd as va
     from vectorizedadjoint_amd import flops as FL
     from vectorizedadjoint_amd import inputs as I
+    if a.workload == "vdp_obs":
+        return bench_observed(a, torch, va, I)
 
     w = I.WORKLOADS["glv10"]
     B = w.batch
