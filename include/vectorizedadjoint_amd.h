@@ -48,8 +48,10 @@ typedef enum va_system {
     VA_SYSTEM_OSCILLATOR = 2,          /* damped oscillator, p = (k, c)                         */
     VA_SYSTEM_OSCILLATOR_FIXED_K = 3,  /* examples/HarmonicOscillator/main.cpp:25-29; p = (mu),
                                           k = va_problem.system_constant                      */
-    VA_SYSTEM_LOTKA_VOLTERRA = 4       /* examples/GeneralizedLotkaVolterra/main.cpp:105-119;
+    VA_SYSTEM_LOTKA_VOLTERRA = 4,      /* examples/GeneralizedLotkaVolterra/main.cpp:105-119;
                                           p = (r[N], A[N][N] row-major), P = N + N*N          */
+    VA_SYSTEM_LOTKA_VOLTERRA_SHARED = 5 /* same system with A shared by the batch: p = r[N],
+                                          P = N; A passed to va_solve_shared; N = 128     */
 } va_system;
 
 /* Tableaus: ButcherTable.hpp:39-251 (Euler, RK4, Cash-Karp 5(4), Fehlberg 7(8))
@@ -175,6 +177,18 @@ va_status va_solve_observed(va_handle *h, const double *y0, const double *p, con
                             const double *obs_y, double *J, double *yT, double *dJdy0,
                             double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
                             int32_t *lane_status, void *stream);
+
+/* Forward + terminal cost + adjoint for VA_SYSTEM_LOTKA_VOLTERRA_SHARED (no reference
+ * counterpart: the reference's parameters are per instance, examples/
+ * GeneralizedLotkaVolterra/main.cpp:105-119).  p = r [N][B]; p_shared = A [N][N]
+ * row-major (device), common to the batch.  Outputs as va_solve, dJdp = dJ/dr
+ * [N][B], and dJdp_shared [N][N] = sum over the instances whose forward ended OK
+ * of dJ/dA (summed in a fixed order).  The stage right-hand sides and adjoints run
+ * as FP64 MFMA products over tiles of 16 instances. */
+va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const double *p_shared,
+                          int32_t cost, const double *cost_w, double *J, double *yT,
+                          double *dJdy0, double *dJdp, double *dJdp_shared, int32_t *n_accepted,
+                          int32_t *n_rejected, int32_t *lane_status, void *stream);
 
 /* Driver::GetT/GetTime/GetDt/GetState (lib/include/Driver.hpp:53-65,
  * StateStorage.hpp:17-42): copies the rows stored for one instance by the last

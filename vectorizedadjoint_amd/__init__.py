@@ -29,7 +29,7 @@ ABI_VERSION = 2
 
 C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
-    "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_trajectory",
+    "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_solve_shared", "va_trajectory",
     "va_max_steps_seen", "va_last_error",
 )
 
@@ -81,11 +81,12 @@ def lib() -> C.CDLL:
         L.va_solve.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.va_max_steps_seen.argtypes = [vp, C.POINTER(C.c_int32), vp]
         L.va_solve_observed.argtypes = [vp] * 13
+        L.va_solve_shared.argtypes = [vp] * 4 + [C.c_int32] + [vp] * 10
         L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
         L.va_last_error.argtypes = []
         L.va_last_error.restype = C.c_char_p
         for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
-                  "va_solve_observed", "va_trajectory", "va_max_steps_seen"):
+                  "va_solve_observed", "va_solve_shared", "va_trajectory", "va_max_steps_seen"):
             getattr(L, f).restype = C.c_int
         _lib_handle = L
     return _lib_handle
@@ -281,6 +282,35 @@ class Solver:
             self._h, _ptr(y0), _ptr(p), _ptr(obs_t), _ptr(obs_y), _ptr(out["J"]),
             _ptr(out["yT"]), _ptr(out["dJdy0"]), _ptr(out["dJdp"]), _ptr(out.get("n_acc")),
             _ptr(out.get("n_rej")), _ptr(out.get("status")), _stream(stream)))
+        self.generation += 1
+        return out
+
+    def solve_shared(self, y0, r, A, cost, cost_weights=None, out=None, stream=None):
+        """Lotka-Volterra with A [N, N] shared by the batch: forward, terminal cost,
+        adjoint.  Returns device tensors J, yT, dJdy0, dJdr [N, B] and dJdA [N, N] (summed
+        over the instances whose forward ended OK)."""
+        import torch
+        N, B = self.problem.n_state, self.problem.batch
+        dev = y0.device
+        if out is None:
+            out = {
+                "J": torch.empty(B, dtype=torch.float64, device=dev),
+                "yT": torch.empty((N, B), dtype=torch.float64, device=dev),
+                "dJdy0": torch.empty((N, B), dtype=torch.float64, device=dev),
+                "dJdr": torch.empty((N, B), dtype=torch.float64, device=dev),
+                "dJdA": torch.empty((N, N), dtype=torch.float64, device=dev),
+                "n_acc": torch.empty(B, dtype=torch.int32, device=dev),
+                "n_rej": torch.empty(B, dtype=torch.int32, device=dev),
+                "status": torch.empty(B, dtype=torch.int32, device=dev),
+            }
+        self._shape_check(y0, (N, B), "f64")
+        self._shape_check(r, (N, B), "f64")
+        self._shape_check(A, (N, N), "f64")
+        _check(lib().va_solve_shared(
+            self._h, _ptr(y0), _ptr(r), _ptr(A), int(cost), _ptr(cost_weights), _ptr(out["J"]),
+            _ptr(out["yT"]), _ptr(out["dJdy0"]), _ptr(out["dJdr"]), _ptr(out["dJdA"]),
+            _ptr(out.get("n_acc")), _ptr(out.get("n_rej")), _ptr(out.get("status")),
+            _stream(stream)))
         self.generation += 1
         return out
 

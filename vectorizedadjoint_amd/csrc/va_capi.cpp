@@ -172,6 +172,9 @@ struct va_handle {
     int32_t obs_ckpt = 0, obs_slots = 0, obs_stride_max = 0;
     double *obs_ck = nullptr, *obs_seg = nullptr;
     int32_t *obs_stride = nullptr, *obs_row = nullptr;
+    // shared-parameter solves
+    int shared_grid = 0;
+    double *abar_part = nullptr;
     int32_t *host_counter = nullptr;
     bool have_forward = false;
 };
@@ -221,6 +224,9 @@ va_status check_handle(const va_handle *h)
 // Handles that keep checkpoints have no per-step store: only va_solve_observed.
 va_status check_rows(const va_handle *h)
 {
+    if (!h->ks.fwd)
+        return fail(VA_ERR_UNSUPPORTED,
+                    "this handle's system has shared parameters: use va_solve_shared");
     if (h->obs_ckpt)
         return fail(VA_ERR_STATE,
                     "this handle keeps checkpoints instead of a trajectory store: use "
@@ -248,6 +254,7 @@ void release(va_handle *h)
     hipFree(h->obs_seg);
     hipFree(h->obs_stride);
     hipFree(h->obs_row);
+    hipFree(h->abar_part);
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
@@ -470,6 +477,9 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (pb->n_param != ks.np)
         return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
                     pb->system, pb->n_state, ks.np, pb->n_param);
+    if (ks.layout == 2 && (pb->mode != VA_MODE_PERSISTENT || pb->n_obs > 0))
+        return fail(VA_ERR_UNSUPPORTED,
+                    "shared-parameter systems run in the persistent mode without observations");
     if (pb->n_obs > 0 && (!ks.obs_fwd || pb->mode != VA_MODE_PERSISTENT))
         return fail(VA_ERR_UNSUPPORTED,
                     "observed solves need an adaptive error stepper on a one-lane system "
@@ -492,7 +502,11 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(VA_ERR_INVALID, "block_size must be a multiple of 64 and <= 256");
     }
     const int64_t B = pb->batch;
-    if (pb->lanes > 0) {
+    if (ks.layout == 2) {
+        // shared-parameter tiles: one workgroup per CU (the matrix fills the LDS)
+        const int64_t ntiles = (B + ks.shared_tile - 1) / ks.shared_tile;
+        h->shared_grid = (int)std::min<int64_t>(ntiles, h->cu_count);
+    } else if (pb->lanes > 0) {
         h->lanes_fwd = h->lanes_rev = pb->lanes;
     } else {
         // every resident instance slot of the device
@@ -579,6 +593,8 @@ va_status va_create(const va_problem *pb, va_handle **out)
                   sizeof(double) * (size_t)rev_threads * h->obs_stride_max * (ks.n + 1));
         }
     }
+    if (ks.layout == 2)
+        alloc((void **)&h->abar_part, sizeof(double) * (size_t)h->shared_grid * ks.n * ks.n);
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {
@@ -776,12 +792,70 @@ va_status va_solve_observed(va_handle *h, const double *y0, const double *p, con
     return copy_out(h, n_accepted, n_rejected, lane_status, s);
 }
 
+va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const double *p_shared,
+                          int32_t cost, const double *cost_w, double *J, double *yT,
+                          double *dJdy0, double *dJdp, double *dJdp_shared, int32_t *n_accepted,
+                          int32_t *n_rejected, int32_t *lane_status, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (h->ks.layout != 2)
+        return fail(VA_ERR_STATE, "va_solve_shared needs a shared-parameter system");
+    if (!y0 || !p || !p_shared || !J || !yT || !dJdy0 || !dJdp || !dJdp_shared)
+        return fail(VA_ERR_INVALID,
+                    "y0, p, p_shared, J, yT, dJdy0, dJdp, dJdp_shared are required");
+    if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
+        return fail(VA_ERR_INVALID, "unknown cost %d", cost);
+    if (cost == VA_COST_LINEAR && !cost_w)
+        return fail(VA_ERR_INVALID, "VA_COST_LINEAR needs cost_weights");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = h->pb.batch;
+    const int n = h->ks.n;
+    if (yT != y0) {
+        hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)n * B, 256)), dim3(256), 0, s, yT,
+                           y0, (int64_t)n * B);
+        VA_HIP(hipGetLastError());
+    }
+    if ((st = fill_i32(h->counters, 0, 4, s))) return st;
+    va::SharedFwdArgs f;
+    f.sp = solve_params(h);
+    f.y = yT;
+    f.r = p;
+    f.A = p_shared;
+    f.traj = h->traj;
+    f.n_acc = h->n_acc;
+    f.n_rej = h->n_rej;
+    f.status = h->status;
+    f.max_steps_seen = h->counters + 2;
+    VA_HIP(h->ks.shared_fwd(f, h->shared_grid, s));
+    h->have_forward = true;
+    // J and the seed dJ/dy(T), written into dJdy0 and swept back in place
+    VA_HIP(va::launch_plain_cost(cost, cost_w, yT, J, dJdy0, h->status, B, n, blocks_for(B, 256),
+                                 256, s));
+    if ((st = order_longest_first(h, s))) return st;
+    va::SharedRevArgs r;
+    r.sp = f.sp;
+    r.r = p;
+    r.A = p_shared;
+    r.traj = h->traj;
+    r.n_acc = h->n_acc;
+    r.status = h->status;
+    r.order = h->order;
+    r.lambda = dJdy0;
+    r.dJdy0 = dJdy0;
+    r.dJdr = dJdp;
+    r.abar_part = h->abar_part;
+    VA_HIP(h->ks.shared_rev(r, h->shared_grid, s));
+    VA_HIP(va::launch_glvs_reduce(h->abar_part, h->shared_grid, dJdp_shared, s));
+    return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
 va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
                         int64_t *n_rows, void *stream)
 {
     va_status st = check_handle(h);
     if (st) return st;
-    if ((st = check_rows(h))) return st;
+    if (h->obs_ckpt) return check_rows(h);
     if (!h->have_forward) return fail(VA_ERR_STATE, "va_trajectory needs a va_forward first");
     if (!n_rows || instance < 0 || instance >= h->pb.batch || max_rows < 0 ||
         (max_rows > 0 && !rows))

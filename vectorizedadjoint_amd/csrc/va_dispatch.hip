@@ -11,6 +11,7 @@ bool kernels_oscillator_fixed_k(int tableau, int adaptive, KernelSet *out);
 bool kernels_lotka_volterra_small(int n, int tableau, int adaptive, KernelSet *out);
 bool kernels_lotka_volterra_10(int tableau, int adaptive, int instances_per_wave,
                                KernelSet *out);
+bool kernels_lotka_volterra_shared(int n, int tableau, int adaptive, KernelSet *out);
 
 __global__ void k_plain_finish(double *lambda_out, const double *lambda_in, double *mu,
                                const double *pbar, const int32_t *status, int64_t B, int n,
@@ -70,7 +71,7 @@ hipError_t launch_plain_cost(int cost, const double *cost_w, const double *yT, d
     return hipGetLastError();
 }
 
-enum { SYS_VDP = 1, SYS_OSC = 2, SYS_OSC_K = 3, SYS_GLV = 4 };
+enum { SYS_VDP = 1, SYS_OSC = 2, SYS_OSC_K = 3, SYS_GLV = 4, SYS_GLV_SHARED = 5 };
 
 bool get_kernels(int system, int n, int tableau, int adaptive, int instances_per_wave,
                  KernelSet *out)
@@ -84,6 +85,7 @@ bool get_kernels(int system, int n, int tableau, int adaptive, int instances_per
     case SYS_GLV:
         if (n == 10) return kernels_lotka_volterra_10(tableau, adaptive, instances_per_wave, out);
         return one_lane && kernels_lotka_volterra_small(n, tableau, adaptive, out);
+    case SYS_GLV_SHARED: return kernels_lotka_volterra_shared(n, tableau, adaptive, out);
     default: return false;
     }
 }

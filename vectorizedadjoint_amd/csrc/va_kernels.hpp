@@ -118,6 +118,30 @@ struct ObsRevArgs {
     ObsStore st;
 };
 
+// Lotka-Volterra with A shared by the batch (tiles of instances on MFMA).
+struct SharedFwdArgs {
+    SolveParams sp;
+    double *y;             // [N][B] in y0, out y(T)
+    const double *r;       // [N][B]
+    const double *A;       // [N][N] row-major, shared
+    double *traj;          // [B][cap+1][N+2]
+    int32_t *n_acc, *n_rej, *status;
+    int32_t *max_steps_seen;
+};
+
+struct SharedRevArgs {
+    SolveParams sp;
+    const double *r;
+    const double *A;
+    const double *traj;
+    const int32_t *n_acc, *status;
+    const int32_t *order;  // [B] longest first
+    const double *lambda;  // [N][B] dJ/dy(T) (may alias dJdy0)
+    double *dJdy0;         // [N][B]
+    double *dJdr;          // [N][B]
+    double *abar_part;     // [grid][N][N] per-workgroup partial dJ/dA
+};
+
 struct KernelSet {
     int n, np, s_rev, s_fwd;
     int flop_rhs, flop_vjp;
@@ -134,7 +158,13 @@ struct KernelSet {
     hipError_t (*obs_rev)(const ObsRevArgs &, int grid, int block, hipStream_t);
     int (*occ_obs_fwd)(int block);
     int (*occ_obs_rev)(int block);
+    // shared-parameter kernels (layout 2), one workgroup per CU
+    hipError_t (*shared_fwd)(const SharedFwdArgs &, int grid, hipStream_t);
+    hipError_t (*shared_rev)(const SharedRevArgs &, int grid, hipStream_t);
+    int shared_tile;
 };
+
+hipError_t launch_glvs_reduce(const double *part, int nparts, double *out, hipStream_t s);
 
 // Returns false when (system, n, tableau, adaptive) has no compiled kernels.
 // instances_per_wave: 64 selects one lane per instance; 64/N (or 0 = default)
