@@ -176,6 +176,73 @@ def bench_observed(a, torch, va, I):
     return 0
 
 
+def bench_shared(a, torch, va, I):
+    """Lotka-Volterra N=128 with A shared by B=16384 instances: stage products and
+    adjoints on FP64 MFMA, dJ/dA reduced over the batch."""
+    from vectorizedadjoint_amd import constants as K
+    w = I.WORKLOADS["glv128_shared"]
+    B = w.batch
+    inp = I.make_inputs(w, B)
+    dev = lambda x: torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    y0, r, A = dev(inp["y0"]), dev(inp["p"]), dev(inp["p_shared"].reshape(w.n, w.n))
+    max_steps = 512
+    log(f"[bench] shared-A GLV N={w.n} B={B}")
+    with va.Solver(va.Problem.from_workload(w, max_steps=max_steps)) as s:
+        out = None
+        for _ in range(a.warmup):
+            out = s.solve_shared(y0, r, A, K.COST_SUM, out=out)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            out = s.solve_shared(y0, r, A, K.COST_SUM, out=out)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t) / a.steps
+        st = out["status"].cpu().numpy()
+        n_acc = out["n_acc"].cpu().numpy()
+        n_rej = out["n_rej"].cpu().numpy()
+    n_ok = int((st == 0).sum())
+    N = w.n
+    # MFMA work actually issued: 6 growth products per attempt (5 stages + f(y_{n+1})),
+    # 6 recompute + 6 x 3 adjoint products per reverse step; 2 N^2 FLOP per instance each
+    gemm = 2.0 * N * N
+    attempts = float(np.sum(n_acc + n_rej))
+    steps = float(np.sum(n_acc[st == 0]))
+    fl_issued = gemm * (6.0 * attempts + 24.0 * steps)
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline (oracle, shared A)")
+        import oracle as O
+        nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+        opb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
+                        abs_tol=w.tol, rel_tol=w.tol, max_steps=max_steps)
+        k = min(B, 16 * nthreads)
+        tc = time.perf_counter()
+        O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), w.cost,
+                      p_shared=inp["p_shared"], nthreads=nthreads)
+        dtc = time.perf_counter() - tc
+        cpu = {"value": k / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
+               "sample": f"first {k} of {B} instances, {dtc:.1f} s wall, {nthreads} threads"}
+    res = {
+        "metric": "adjoint gradients/sec, GLV N=128 shared A, B=16384, FP64 MFMA",
+        "value": n_ok / sec, "unit": "gradients/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 inputs, random-init shared A)",
+        "config": {"workload": "glv128_shared", "n_state": N, "batch": B,
+                   "tableau": "cash_karp54", "tol": w.tol, "t_span": [w.t0, w.t1],
+                   "cost": "sum_i x_i(T)", "instances_ok": n_ok, "max_steps": max_steps},
+        "roofline": {"bound": "mfma", "compute": "fp64 MFMA 16x16x4", "kernel": "solve",
+                     "achieved": fl_issued / sec / 1e12, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": fl_issued / sec / 1e12 / FP64_PEAK_TFLOPS,
+                     "traffic": None, "flop_per_solve": fl_issued},
+        "cpu_baseline": cpu,
+        "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
+                               "rejected_mean": float(n_rej.mean())},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,8 +250,9 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form")
-    ap.add_argument("--workload", default="glv10", choices=["glv10", "vdp_obs"],
-                    help="glv10: the headline metric; vdp_obs: the observed Van der Pol solve")
+    ap.add_argument("--workload", default="glv10", choices=["glv10", "vdp_obs", "glv128_shared"],
+                    help="glv10: the headline metric; vdp_obs: the observed Van der Pol solve; "
+                         "glv128_shared: Lotka-Volterra N=128 with a shared A on MFMA")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB five times (the known-size "
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -210,6 +278,8 @@ def main(argv=None) -> int:
     from vectorizedadjoint_amd import inputs as I
     if a.workload == "vdp_obs":
         return bench_observed(a, torch, va, I)
+    if a.workload == "glv128_shared":
+        return bench_shared(a, torch, va, I)
 
     w = I.WORKLOADS["glv10"]
     B = w.batch
