@@ -113,8 +113,13 @@ typedef struct va_problem {
                                         (or, for observed solves, the checkpoint store)
                                         would exceed it                               */
     int32_t n_obs;             /* observation times of va_solve_observed (0 = none)   */
-    int32_t reserved;          /* must be 0                                           */
+    int32_t flags;             /* VA_FLAG_* bits                                      */
 } va_problem;
+
+/* va_problem.flags: va_solve (persistent mode) records its launch sequence once in a
+ * HIP graph per distinct set of argument pointers and replays it on the caller's
+ * stream afterwards (one graph launch instead of about ten kernel launches). */
+#define VA_FLAG_GRAPH 1
 
 typedef struct va_handle va_handle;
 

@@ -44,7 +44,7 @@ class VaProblem(C.Structure):
         ("max_attempts", C.c_int32), ("max_steps", C.c_int32), ("mode", C.c_int32),
         ("block_size", C.c_int32), ("instances_per_wave", C.c_int32),
         ("checkpoint_slots", C.c_int32), ("lanes", C.c_int64),
-        ("trajectory_budget_bytes", C.c_int64), ("n_obs", C.c_int32), ("reserved", C.c_int32),
+        ("trajectory_budget_bytes", C.c_int64), ("n_obs", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -120,6 +120,7 @@ class Problem:
     trajectory_budget_bytes: int = 0
     n_obs: int = 0
     checkpoint_slots: int = 0
+    flags: int = 0
 
     def to_c(self) -> VaProblem:
         pb = VaProblem()
@@ -136,6 +137,7 @@ class Problem:
         pb.instances_per_wave = self.instances_per_wave
         pb.trajectory_budget_bytes = self.trajectory_budget_bytes
         pb.n_obs, pb.checkpoint_slots = self.n_obs, self.checkpoint_slots
+        pb.flags = self.flags
         return pb
 
     @classmethod

@@ -29,6 +29,9 @@ LANE_MAX_STEPS = 3
 MODE_PERSISTENT = 0
 MODE_PLAIN = 1
 
+# va_problem.flags
+FLAG_GRAPH = 1
+
 # va_status
 STATUS_NAMES = {
     0: "VA_OK",

@@ -175,6 +175,11 @@ struct va_handle {
     // shared-parameter solves
     int shared_grid = 0;
     double *abar_part = nullptr;
+    // VA_FLAG_GRAPH: va_solve recorded once per argument set, then replayed
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void *graph_key[12] = {};
+    int32_t graph_cost = 0;
     int32_t *host_counter = nullptr;
     bool have_forward = false;
 };
@@ -255,6 +260,8 @@ void release(va_handle *h)
     hipFree(h->obs_stride);
     hipFree(h->obs_row);
     hipFree(h->abar_part);
+    if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
+    if (h->cap_stream) hipStreamDestroy(h->cap_stream);
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
@@ -460,7 +467,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(VA_ERR_INVALID,
                     "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
     va::KernelSet ks{};
-    if (pb->reserved != 0) return fail(VA_ERR_INVALID, "va_problem.reserved must be 0");
+    if (pb->flags & ~VA_FLAG_GRAPH) return fail(VA_ERR_INVALID, "unknown va_problem.flags bits");
     if (pb->n_obs < 0) return fail(VA_ERR_INVALID, "n_obs must be >= 0");
     if (pb->checkpoint_slots < 0 || pb->checkpoint_slots == 1 || pb->checkpoint_slots % 2 != 0)
         return fail(VA_ERR_INVALID, "checkpoint_slots must be 0 or an even number >= 2");
@@ -686,6 +693,11 @@ va_status va_sensitivity_matrix(va_handle *h, const double *p, double *S, void *
     return st;
 }
 
+static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, int32_t cost,
+                               const double *cost_weights, double *J, double *yT, double *dJdy0,
+                               double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                               int32_t *lane_status, void *stream);
+
 va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost,
                    const double *cost_weights, double *J, double *yT, double *dJdy0,
                    double *dJdp, int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
@@ -694,6 +706,47 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
     va_status st = check_handle(h);
     if (st) return st;
     if ((st = check_rows(h))) return st;
+    if (!(h->pb.flags & VA_FLAG_GRAPH) || h->pb.mode != VA_MODE_PERSISTENT)
+        return solve_enqueue(h, y0, p, cost, cost_weights, J, yT, dJdy0, dJdp, n_accepted,
+                             n_rejected, lane_status, stream);
+    const void *key[12] = {y0, p, cost_weights, J, yT, dJdy0, dJdp, n_accepted, n_rejected,
+                           lane_status, nullptr, nullptr};
+    bool same = h->graph_exec && h->graph_cost == cost;
+    for (int i = 0; i < 12 && same; ++i) same = key[i] == h->graph_key[i];
+    if (!same) {
+        // record the launch sequence on a private stream, then keep the executable graph
+        if (!h->cap_stream) VA_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        if (h->graph_exec) {
+            (void)hipGraphExecDestroy(h->graph_exec);
+            h->graph_exec = nullptr;
+        }
+        VA_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        st = solve_enqueue(h, y0, p, cost, cost_weights, J, yT, dJdy0, dJdp, n_accepted,
+                           n_rejected, lane_status, h->cap_stream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
+        if (st) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        VA_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&h->graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        VA_HIP(ei);
+        for (int i = 0; i < 12; ++i) h->graph_key[i] = key[i];
+        h->graph_cost = cost;
+    }
+    VA_HIP(hipGraphLaunch(h->graph_exec, (hipStream_t)stream));
+    h->have_forward = true;
+    return VA_OK;
+}
+
+static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, int32_t cost,
+                               const double *cost_weights, double *J, double *yT, double *dJdy0,
+                               double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                               int32_t *lane_status, void *stream)
+{
+    va_status st = VA_OK;
     if (!y0 || !p || !J || !yT || !dJdy0 || !dJdp)
         return fail(VA_ERR_INVALID, "y0, p, J, yT, dJdy0, dJdp are required");
     if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
