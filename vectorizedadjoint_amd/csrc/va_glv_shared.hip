@@ -49,6 +49,8 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 }
 
 // G[inst][16w + c] = sum_k X[inst][k] A[16w + c][k] for the tile (X staged in Xs).
+// The forward and the reverse's recompute call this same sequence, so they
+// produce the same bits.
 __device__ __forceinline__ d4 gemm_xat(const double *Xs, const double *As, int w, int r, int c)
 {
     d4 acc = {0.0, 0.0, 0.0, 0.0};
