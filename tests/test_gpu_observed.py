@@ -116,3 +116,26 @@ def test_gradient_against_central_differences():
     rel = np.abs(fd - ad) / np.abs(ad)
     assert np.median(rel) <= 2e-6 and np.percentile(rel, 90) <= 1e-4, (
         np.median(rel), np.percentile(rel, 90))
+
+
+@pytest.mark.parametrize("kw", [{"trajectory_budget_bytes": 8 << 20}, {"checkpoint_slots": 4}])
+def test_terminal_cost_solve_over_budget_uses_checkpoints(kw):
+    """va_solve whose per-step store exceeds the budget (or with forced checkpoints)
+    runs on checkpoints with a recomputing reverse: bit-identical to the full store."""
+    import torch
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import inputs as I
+    w = I.WORKLOADS["vdp"]
+    B = 1024
+    inp = I.make_inputs(w, B)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    y0, p = dev(inp["y0"]), dev(inp["p"])
+    print(f"[gpu] terminal cost over budget {kw}", flush=True)
+    with va.Solver(va.Problem.from_workload(w, batch=B, max_steps=4096)) as full, \
+            va.Solver(va.Problem.from_workload(w, batch=B, max_steps=4096, **kw)) as ck:
+        a = {k: v.cpu().numpy() for k, v in full.solve(y0, p, w.cost).items()}
+        b = {k: v.cpu().numpy() for k, v in ck.solve(y0, p, w.cost).items()}
+        with pytest.raises(va.VaError, match="checkpoints"):
+            ck.forward(y0.clone(), p)
+    for k in a:
+        assert np.array_equal(a[k], b[k], equal_nan=True), k

@@ -471,8 +471,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (pb->n_obs < 0) return fail(VA_ERR_INVALID, "n_obs must be >= 0");
     if (pb->checkpoint_slots < 0 || pb->checkpoint_slots == 1 || pb->checkpoint_slots % 2 != 0)
         return fail(VA_ERR_INVALID, "checkpoint_slots must be 0 or an even number >= 2");
-    if (pb->checkpoint_slots > 0 && pb->n_obs == 0)
-        return fail(VA_ERR_INVALID, "checkpoint_slots applies to observed solves (n_obs > 0)");
+
     if (pb->instances_per_wave < 0 || pb->instances_per_wave > 64)
         return fail(VA_ERR_INVALID, "instances_per_wave must be in [0, 64]");
     if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive,
@@ -487,10 +486,11 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (ks.layout == 2 && (pb->mode != VA_MODE_PERSISTENT || pb->n_obs > 0))
         return fail(VA_ERR_UNSUPPORTED,
                     "shared-parameter systems run in the persistent mode without observations");
-    if (pb->n_obs > 0 && (!ks.obs_fwd || pb->mode != VA_MODE_PERSISTENT))
+    const bool obs_capable = ks.obs_fwd && pb->mode == VA_MODE_PERSISTENT;
+    if ((pb->n_obs > 0 || pb->checkpoint_slots > 0) && !obs_capable)
         return fail(VA_ERR_UNSUPPORTED,
-                    "observed solves need an adaptive error stepper on a one-lane system "
-                    "(Van der Pol, oscillators) in the persistent mode");
+                    "observed and checkpointed solves need an adaptive error stepper on a "
+                    "one-lane system (Van der Pol, oscillators) in the persistent mode");
 
     va_handle *h = new (std::nothrow) va_handle();
     if (!h) return fail(VA_ERR_OUT_OF_MEMORY, "host allocation");
@@ -523,8 +523,17 @@ va_status va_create(const va_problem *pb, va_handle **out)
     }
     const double budget = (double)pb->trajectory_budget_bytes;
     bool full_store = true;  // per-step rows in h->traj / h->ptraj
-    if (pb->n_obs > 0) {
-        // observation handles: full rows while they fit the budget, else checkpoints
+    const size_t rows_b = (size_t)pb->max_steps + 1;
+    const double store =
+        (double)sizeof(double) * (double)B *
+        (double)(pb->mode == VA_MODE_PERSISTENT
+                     ? rows_b * (ks.n + 2)
+                     : rows_b * (2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n));
+    const bool over = budget > 0.0 && store > budget;
+    if (pb->n_obs > 0 || (obs_capable && (over || pb->checkpoint_slots > 0))) {
+        // observation handles, and terminal-cost handles whose store exceeds the
+        // budget: full rows while they fit, else checkpoints (observation kernels
+        // with no observations and the terminal cost)
         const int64_t per_block = h->block;
         h->lanes_obs_fwd = (int64_t)std::max(1, ks.occ_obs_fwd(h->block)) * h->cu_count * per_block;
         h->lanes_obs_rev = (int64_t)std::max(1, ks.occ_obs_rev(h->block)) * h->cu_count * per_block;
@@ -563,19 +572,11 @@ va_status va_create(const va_problem *pb, va_handle **out)
             h->obs_slots = slots;
             h->obs_stride_max = sm;
         }
-    } else {
-        const size_t rows_b = (size_t)pb->max_steps + 1;
-        const size_t per_inst =
-            pb->mode == VA_MODE_PERSISTENT
-                ? rows_b * (ks.n + 2)
-                : rows_b * (2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n);
-        const double store = (double)sizeof(double) * (double)per_inst * (double)B;
-        if (budget > 0.0 && store > budget) {
-            delete h;
-            return fail(VA_ERR_OUT_OF_MEMORY,
-                        "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
-                        store, (long long)pb->trajectory_budget_bytes);
-        }
+    } else if (over) {
+        delete h;
+        return fail(VA_ERR_OUT_OF_MEMORY,
+                    "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
+                    store, (long long)pb->trajectory_budget_bytes);
     }
 
     const size_t rows = (size_t)pb->max_steps + 1;
@@ -589,8 +590,8 @@ va_status va_create(const va_problem *pb, va_handle **out)
     alloc((void **)&h->counters, sizeof(int32_t) * 4);
     alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
-    if (pb->n_obs > 0) {
-        alloc((void **)&h->obs_row, sizeof(int32_t) * (size_t)B * pb->n_obs);
+    if (pb->n_obs > 0 || h->obs_ckpt) {
+        alloc((void **)&h->obs_row, sizeof(int32_t) * std::max<size_t>(1, (size_t)B * pb->n_obs));
         if (h->obs_ckpt) {
             const int64_t rev_threads =
                 (int64_t)blocks_for(std::min<int64_t>(h->lanes_obs_rev, B), h->block) * h->block;
@@ -697,6 +698,11 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
                                const double *cost_weights, double *J, double *yT, double *dJdy0,
                                double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
                                int32_t *lane_status, void *stream);
+static va_status observed_enqueue(va_handle *h, const double *y0, const double *p,
+                                  const double *obs_t, const double *obs_y, int32_t cost,
+                                  const double *cost_w, double *J, double *yT, double *dJdy0,
+                                  double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                                  int32_t *lane_status, void *stream);
 
 va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost,
                    const double *cost_weights, double *J, double *yT, double *dJdy0,
@@ -705,6 +711,17 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
 {
     va_status st = check_handle(h);
     if (st) return st;
+    if (h->obs_ckpt && h->pb.n_obs == 0) {
+        // terminal cost over a checkpointed trajectory
+        if (!y0 || !p || !J || !yT || !dJdy0 || !dJdp)
+            return fail(VA_ERR_INVALID, "y0, p, J, yT, dJdy0, dJdp are required");
+        if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
+            return fail(VA_ERR_INVALID, "unknown cost %d", cost);
+        if (cost == VA_COST_LINEAR && !cost_weights)
+            return fail(VA_ERR_INVALID, "VA_COST_LINEAR needs cost_weights");
+        return observed_enqueue(h, y0, p, nullptr, nullptr, cost, cost_weights, J, yT, dJdy0,
+                                dJdp, n_accepted, n_rejected, lane_status, stream);
+    }
     if ((st = check_rows(h))) return st;
     if (!(h->pb.flags & VA_FLAG_GRAPH) || h->pb.mode != VA_MODE_PERSISTENT)
         return solve_enqueue(h, y0, p, cost, cost_weights, J, yT, dJdy0, dJdp, n_accepted,
@@ -779,17 +796,16 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
     return copy_out(h, n_accepted, n_rejected, lane_status, s);
 }
 
-va_status va_solve_observed(va_handle *h, const double *y0, const double *p, const double *obs_t,
-                            const double *obs_y, double *J, double *yT, double *dJdy0,
-                            double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
-                            int32_t *lane_status, void *stream)
+// Observation-kernel solve: the running cost over obs (K = pb.n_obs), plus the
+// terminal cost when cost > 0 (K = 0 serves terminal-cost handles that keep
+// checkpoints).
+static va_status observed_enqueue(va_handle *h, const double *y0, const double *p,
+                                  const double *obs_t, const double *obs_y, int32_t cost,
+                                  const double *cost_w, double *J, double *yT, double *dJdy0,
+                                  double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                                  int32_t *lane_status, void *stream)
 {
-    va_status st = check_handle(h);
-    if (st) return st;
-    if (h->pb.n_obs <= 0)
-        return fail(VA_ERR_STATE, "va_solve_observed needs a handle created with n_obs > 0");
-    if (!y0 || !p || !obs_t || !obs_y || !J || !yT || !dJdy0 || !dJdp)
-        return fail(VA_ERR_INVALID, "y0, p, obs_t, obs_y, J, yT, dJdy0, dJdp are required");
+    va_status st = VA_OK;
     hipStream_t s = (hipStream_t)stream;
     const int64_t B = h->pb.batch;
     const int n = h->ks.n;
@@ -839,10 +855,28 @@ va_status va_solve_observed(va_handle *h, const double *y0, const double *p, con
     r.dJdp = dJdp;
     r.counter = h->counters + 1;
     r.order = h->order;
+    r.cost = cost;
+    r.cost_w = cost_w;
+    r.J = J;
     r.st = os;
     VA_HIP(h->ks.obs_rev(r, blocks_for(std::min<int64_t>(h->lanes_obs_rev, B), h->block),
                          h->block, s));
     return copy_out(h, n_accepted, n_rejected, lane_status, s);
+}
+
+va_status va_solve_observed(va_handle *h, const double *y0, const double *p, const double *obs_t,
+                            const double *obs_y, double *J, double *yT, double *dJdy0,
+                            double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                            int32_t *lane_status, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (h->pb.n_obs <= 0)
+        return fail(VA_ERR_STATE, "va_solve_observed needs a handle created with n_obs > 0");
+    if (!y0 || !p || !obs_t || !obs_y || !J || !yT || !dJdy0 || !dJdp)
+        return fail(VA_ERR_INVALID, "y0, p, obs_t, obs_y, J, yT, dJdy0, dJdp are required");
+    return observed_enqueue(h, y0, p, obs_t, obs_y, 0, nullptr, J, yT, dJdy0, dJdp, n_accepted,
+                            n_rejected, lane_status, stream);
 }
 
 va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const double *p_shared,

@@ -115,6 +115,9 @@ struct ObsRevArgs {
     double *dJdp;          // [P][B]
     int32_t *counter;
     const int32_t *order;
+    int32_t cost;          // > 0: terminal cost added to the seed, J written
+    const double *cost_w;
+    double *J;
     ObsStore st;
 };
 

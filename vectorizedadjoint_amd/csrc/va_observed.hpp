@@ -295,6 +295,7 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
             lam[i] = 0.0;
             yT[i] = a.yT[(int64_t)i * B + b];
         }
+        if (a.cost) a.J[b] = terminal_cost<N>(a.cost, a.cost_w, yT, lam);
 #pragma unroll
         for (int k = 0; k < P; ++k) pbar[k] = 0.0;
         const int32_t cnt = a.n_acc[b];
