@@ -37,13 +37,16 @@ def log(msg):
 
 
 def time_solver(solver, y0, p, cost, steps, warmup, torch):
+    """Seconds per solve.  p is one parameter tensor, or a list cycled over the
+    warmup and timed solves (a parameter set that changes at every step)."""
+    ps = p if isinstance(p, list) else [p]
     out = None
-    for _ in range(warmup):
-        out = solver.solve(y0, p, cost, out=out)
+    for k in range(warmup):
+        out = solver.solve(y0, ps[k % len(ps)], cost, out=out)
     torch.cuda.synchronize()
     t = time.perf_counter()
-    for _ in range(steps):
-        out = solver.solve(y0, p, cost, out=out)
+    for k in range(steps):
+        out = solver.solve(y0, ps[(warmup + k) % len(ps)], cost, out=out)
     torch.cuda.synchronize()
     return (time.perf_counter() - t) / steps, out
 
@@ -295,6 +298,25 @@ def main(argv=None) -> int:
         st = out["status"].cpu().numpy()
         tf, tr, n_acc, n_rej = kernel_times(s, y0, p, 5, torch)
     n_ok = int((st == 0).sum())
+    # The schedule: by default each forward takes instances longest-first by the
+    # previous solve's attempt counts.  Also timed: index order in every solve
+    # (cold), and a parameter set that drifts at every step by a relative 1e-6
+    # (an optimizer loop), where the previous solve's counts are a prediction.
+    log("[bench] schedule variants: index order; drifting parameters")
+    with va.Solver(va.Problem.from_workload(w, max_steps=MAX_STEPS,
+                                            flags=va.FLAG_COLD_ORDER)) as s:
+        sec_cold, out_c = time_solver(s, y0, p, w.cost, a.steps, a.warmup, torch)
+        n_ok_cold = int((out_c["status"] == 0).sum().item())
+    gen = torch.Generator(device="cuda").manual_seed(12345)
+    p_drift, pk = [], p.clone()
+    for _ in range(a.steps + a.warmup):
+        pk = pk * (1.0 + 1e-6 * torch.randn(pk.shape, dtype=torch.float64, device="cuda",
+                                            generator=gen))
+        p_drift.append(pk)
+    with va.Solver(pb) as s:
+        sec_drift, out_d = time_solver(s, y0, p_drift, w.cost, a.steps, a.warmup, torch)
+        n_ok_drift = int((out_d["status"] == 0).sum().item())
+    del p_drift
     fl_f = FL.forward_flops(w.system, w.n, w.tableau, n_acc, n_rej)
     # the reverse sweeps only instances whose forward ended OK
     fl_r = FL.reverse_flops(w.system, w.n, w.tableau, n_acc[st == 0])
@@ -338,6 +360,11 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "plain_gpu": plain,
         "speedup_vs_plain": (plain["ms_per_step"] / (sec * 1e3)) if plain else None,
+        "schedule": {"default": "forward longest-first by the previous solve's attempts",
+                     "cold_index_order": {"ms_per_step": sec_cold * 1e3,
+                                          "value": n_ok_cold / sec_cold},
+                     "drifting_parameters_1e-6": {"ms_per_step": sec_drift * 1e3,
+                                                  "value": n_ok_drift / sec_drift}},
         "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
                                "accepted_max": int(n_acc.max()),
                                "rejected_mean": float(n_rej.mean())},

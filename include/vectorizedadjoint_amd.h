@@ -120,6 +120,12 @@ typedef struct va_problem {
  * HIP graph per distinct set of argument pointers and replays it on the caller's
  * stream afterwards (one graph launch instead of about ten kernel launches). */
 #define VA_FLAG_GRAPH 1
+/* va_problem.flags: by default a persistent forward whose batch exceeds the instances
+ * in flight takes its instances longest-first by the attempt counts of the handle's
+ * previous forward (index order the first time), so that instances that ran long
+ * last time start first; results do not depend on the order.  With this flag every
+ * forward takes instances in index order. */
+#define VA_FLAG_COLD_ORDER 2
 
 typedef struct va_handle va_handle;
 

@@ -104,8 +104,8 @@ def lib():
         L.orc_get_tableau.argtypes = [C.c_int, C.POINTER(Tableau)]
         L.orc_rhs.argtypes = [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, dp]
         L.orc_vjp.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.c_double, dp, dp, dp]
-        L.orc_pow.argtypes = [C.c_double, C.c_double]
-        L.orc_pow.restype = C.c_double
+        L.orc_root.argtypes = [C.c_double, C.c_int]
+        L.orc_root.restype = C.c_double
         L.orc_rk_step.argtypes = [C.POINTER(Problem), dp, dp, C.c_double, C.c_double, dp, dp]
         L.orc_traj_alloc.argtypes = [C.POINTER(Trajectory), C.c_int64, C.c_int]
         L.orc_traj_free.argtypes = [C.POINTER(Trajectory)]
@@ -216,8 +216,9 @@ def vjp(system, x, p, w, t=0.0, sys_const=0.0):
     return wjx, wjp
 
 
-def pow_(x, y):
-    return lib().orc_pow(float(x), float(y))
+def root_(x, q):
+    """x^(-1/q) as the step controller computes it (q = 2..8)."""
+    return lib().orc_root(float(x), int(q))
 
 
 def rk_step(pb, y, p, t, h):

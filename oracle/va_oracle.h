@@ -27,9 +27,10 @@
  * an explicit fma() and the file is compiled with -ffp-contract=off, so the
  * exact IEEE operation sequence is fixed and documented; the HIP kernels
  * follow the same sequence, which makes forward trajectories comparable bit
- * for bit.  The controller's pow() is orc_pow(): natural log by the atanh
- * series and exp by Cody-Waite reduction plus a Taylor polynomial (see
- * va_oracle.c), within a few ulp of the C library pow.
+ * for bit.  The controller's pow(err, -1/q) is orc_root(err, q): a polynomial
+ * start and two Newton steps for the q-th root (see va_oracle.c), within about
+ * one ulp of the exact root and 1.5 ulp of the C library pow on the
+ * controller's domain.
  *
  * How each part is pinned (tests/ file: what fixes it independently):
  *   tableaus ........ test_oracle_tableau: rooted-tree order conditions to the
@@ -155,8 +156,8 @@ void orc_rhs(int system, int n, const double *x, const double *p, double t, doub
 void orc_vjp(int system, int n, int np, const double *x, const double *p, double t,
              double sys_const, const double *w, double *wJx, double *wJp);
 
-/* pow(x, y) as used by the step controller (x >= 0). */
-double orc_pow(double x, double y);
+/* x^(-1/q) for the step controller, q = 2..8 (NaN for other q). */
+double orc_root(double x, int q);
 
 /* One explicit RK step without error control: out = y + h sum b_m K_m. */
 void orc_rk_step(const orc_problem *pb, const double *y, const double *p, double t, double h,

@@ -1,9 +1,10 @@
-"""Pins the oracle's building blocks: the controller pow against the C library
-pow, the right-hand sides against the reference example functors, and the
+"""Pins the oracle's building blocks: the controller root x^(-1/q) against the
+exact root and the C library pow, the right-hand sides against the reference example functors, and the
 vector-Jacobian products against complex-step derivatives (exact to rounding
 for these polynomial systems)."""
 import math
 import os
+from decimal import Decimal, getcontext
 
 import numpy as np
 import pytest
@@ -11,6 +12,7 @@ import pytest
 import oracle as O
 from refmath import read_alphas_csv, rhs
 
+getcontext().prec = 40
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
@@ -18,29 +20,39 @@ def ulp_diff(a, b):
     return abs(a - b) / math.ulp(b) if b != 0 else abs(a)
 
 
-@pytest.mark.parametrize("e", [-1 / 3, -1 / 5, -1 / 6, -1 / 7, -1 / 8])
-def test_pow_matches_libm_on_controller_domain(e):
-    # the controller raises err in [5^-8, ~1e6] to -1/q (q = 4, 5, 7, 8 error/stepper orders)
-    xs = np.concatenate([np.geomspace(5.0 ** -8, 1e6, 4001), np.linspace(0.01, 3.0, 2001),
-                         [5.0 ** -5, 5.0 ** -8, 0.5, 1.0, 1.0 + 2 ** -52, 1.0 - 2 ** -53]])
-    worst = max(ulp_diff(O.pow_(x, e), x ** e) for x in xs)
-    assert worst <= 6.0, worst
+def exact_root(x, q):
+    """x^(-1/q) to 40 digits, rounded once to a double."""
+    return float(Decimal(x) ** (Decimal(-1) / Decimal(q)))
 
 
-@pytest.mark.parametrize("e", [-1 / 3, -1 / 8, 0.5, 2.0])
-def test_pow_relative_accuracy_wide_range(e):
-    # outside that domain the exponent's rounding is amplified by |e log x|; still 1e-14
-    xs = np.geomspace(1e-100, 1e100, 4001)
-    worst = max(abs(O.pow_(x, e) - x ** e) / x ** e for x in xs)
-    assert worst <= 1e-13, worst
+@pytest.mark.parametrize("q", [2, 3, 4, 5, 6, 7, 8])
+def test_root_against_exact_and_libm_on_controller_domain(q):
+    # the controller raises err in [5^-8, ~1e6] to -1/q (q = ORDER or ERR_ORDER - 1)
+    xs = np.concatenate([np.geomspace(5.0 ** -8, 1e6, 3001), np.linspace(0.01, 3.0, 1501),
+                         [5.0 ** -5, 5.0 ** -8, 0.5, 1.0, 1.0 + 2 ** -52, 1.0 - 2 ** -53,
+                          2.0, 2.0 ** q, 2.0 ** q - 2 ** -40]])
+    worst_exact = max(ulp_diff(O.root_(x, q), exact_root(x, q)) for x in xs)
+    worst_libm = max(ulp_diff(O.root_(x, q), x ** (-1.0 / q)) for x in xs)
+    assert worst_exact <= 1.0, worst_exact
+    assert worst_libm <= 4.0, worst_libm
 
 
-def test_pow_special_values():
-    assert O.pow_(0.0, -0.2) == math.inf
-    assert O.pow_(math.inf, -0.2) == 0.0
-    assert math.isnan(O.pow_(math.nan, -0.2))
-    assert O.pow_(1e300, -1 / 3) == pytest.approx(1e-100, rel=1e-14)
-    assert O.pow_(1.0, -1 / 3) == 1.0
+@pytest.mark.parametrize("q", [2, 3, 5, 8])
+def test_root_full_range_and_subnormals(q):
+    xs = np.concatenate([np.geomspace(1e-307, 1e307, 2001),
+                         [5e-324, 1e-320, 2.2250738585072014e-308, 1.7976931348623157e308]])
+    worst = max(ulp_diff(O.root_(x, q), exact_root(x, q)) for x in xs)
+    assert worst <= 1.0, worst
+
+
+def test_root_special_values():
+    assert O.root_(0.0, 3) == math.inf
+    assert O.root_(math.inf, 3) == 0.0
+    assert math.isnan(O.root_(math.nan, 5))
+    assert math.isnan(O.root_(-1.0, 5))
+    assert math.isnan(O.root_(2.0, 9))
+    assert O.root_(8.0, 3) == 0.5 and O.root_(2.0 ** -40, 8) == 32.0
+    assert abs(O.root_(1.0, 5) - 1.0) <= 2 ** -53
 
 
 def _systems():

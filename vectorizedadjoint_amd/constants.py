@@ -31,6 +31,7 @@ MODE_PLAIN = 1
 
 # va_problem.flags
 FLAG_GRAPH = 1
+FLAG_COLD_ORDER = 2  # every forward takes instances in index order (no schedule reuse)
 
 # va_status
 STATUS_NAMES = {

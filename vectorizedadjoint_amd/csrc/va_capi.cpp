@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 #include <new>
 
 #include "../../include/vectorizedadjoint_amd.h"
@@ -85,29 +86,35 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
     out[idx] = ptraj[((size_t)row * fields + f) * (size_t)B + b];
 }
 
-// Longest-first order of the reverse sweep.  The reverse of one instance is a
-// chain of n_acc dependent steps; when the long chains start last they set the
-// end of the launch.  Instances are bucketed by accepted-step count (buckets of
-// 16 steps below 512, of 256 steps above, the last open-ended) and the reverse
-// takes them bucket by bucket from the longest; within a bucket the order is
-// arbitrary.  Each instance's result does not depend on the order.
+// Longest-first orders.  The reverse of one instance is a chain of n_acc
+// dependent steps, its forward a chain of n_acc + n_rej attempts; when the long
+// chains start last they set the end of the launch.  Instances are bucketed by
+// the chain length (buckets of 16 below 512, of 256 above, the last
+// open-ended) and taken bucket by bucket from the longest; within a bucket the
+// order is arbitrary.  Reverse order (mode 0): accepted steps, failed instances
+// last (they only write NaN).  Forward order (mode 1): attempts, every status.
+// Each instance's result does not depend on the order.
 constexpr int VA_ORDER_BUCKETS = 64;
 
-__device__ __forceinline__ int order_bucket(int32_t n_acc, int32_t status)
+__device__ __forceinline__ int order_bucket(const int32_t *n_acc, const int32_t *n_rej,
+                                            const int32_t *status, int64_t b, int mode)
 {
-    if (status != 0) return 0;  // failed instances only write NaN: last
-    const int k = n_acc < 512 ? (n_acc >> 4) : 32 + ((n_acc - 512) >> 8);
+    int32_t len = n_acc[b];
+    if (mode == 0 && status[b] != 0) return 0;
+    if (mode == 1) len += n_rej[b];
+    const int k = len < 512 ? (len >> 4) : 32 + ((len - 512) >> 8);
     return k < VA_ORDER_BUCKETS - 1 ? k : VA_ORDER_BUCKETS - 1;
 }
 
-__global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const int32_t *status,
-                                                    int64_t B, int32_t *hist)
+__global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const int32_t *n_rej,
+                                                    const int32_t *status, int64_t B, int mode,
+                                                    int32_t *hist)
 {
     __shared__ int32_t cnt[VA_ORDER_BUCKETS];
     if (threadIdx.x < VA_ORDER_BUCKETS) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < B) atomicAdd(&cnt[order_bucket(n_acc[b], status[b])], 1);
+    if (b < B) atomicAdd(&cnt[order_bucket(n_acc, n_rej, status, b, mode)], 1);
     __syncthreads();
     if (threadIdx.x < VA_ORDER_BUCKETS && cnt[threadIdx.x])
         atomicAdd(&hist[threadIdx.x], cnt[threadIdx.x]);
@@ -126,8 +133,9 @@ __global__ void k_order_offsets(const int32_t *hist, int32_t *cursor)
 }
 
 __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *n_acc,
+                                                       const int32_t *n_rej,
                                                        const int32_t *status, int64_t B,
-                                                       int32_t *cursor, int32_t *order)
+                                                       int mode, int32_t *cursor, int32_t *order)
 {
     __shared__ int32_t cnt[VA_ORDER_BUCKETS], base[VA_ORDER_BUCKETS];
     if (threadIdx.x < VA_ORDER_BUCKETS) cnt[threadIdx.x] = 0;
@@ -135,7 +143,7 @@ __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *n_acc,
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int k = 0, r = 0;
     if (b < B) {
-        k = order_bucket(n_acc[b], status[b]);
+        k = order_bucket(n_acc, n_rej, status, b, mode);
         r = atomicAdd(&cnt[k], 1);
     }
     __syncthreads();
@@ -166,6 +174,7 @@ struct va_handle {
     int32_t *n_acc = nullptr, *n_rej = nullptr, *status = nullptr;
     int32_t *counters = nullptr;  // [0] fwd queue, [1] rev queue, [2] max steps, [3] active
     int32_t *order = nullptr;     // [B] reverse order, longest first
+    int32_t *order_fwd = nullptr; // [B] next forward's order (previous attempts, longest first)
     int32_t *order_tmp = nullptr; // [2 * VA_ORDER_BUCKETS] histogram, cursors
     // observed solves
     int64_t lanes_obs_fwd = 0, lanes_obs_rev = 0;
@@ -254,6 +263,7 @@ void release(va_handle *h)
     hipFree(h->status);
     hipFree(h->counters);
     hipFree(h->order);
+    hipFree(h->order_fwd);
     hipFree(h->order_tmp);
     hipFree(h->obs_ck);
     hipFree(h->obs_seg);
@@ -265,21 +275,32 @@ void release(va_handle *h)
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
-// Longest-first order over the instances of the last forward (into h->order).
-va_status order_longest_first(va_handle *h, hipStream_t s)
+// Longest-first order over the instances of the last forward: mode 0 into
+// h->order (the reverse), mode 1 into h->order_fwd (the next forward).
+va_status order_by_length(va_handle *h, int mode, hipStream_t s)
 {
     const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->order_tmp, 0, 2 * VA_ORDER_BUCKETS, s);
     if (st) return st;
     const int gb = blocks_for(B, 256);
-    hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
-                       h->order_tmp);
+    int32_t *out = mode == 0 ? h->order : h->order_fwd;
+    hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej, h->status,
+                       B, mode, h->order_tmp);
     hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
                        h->order_tmp + VA_ORDER_BUCKETS);
-    hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->status, B,
-                       h->order_tmp + VA_ORDER_BUCKETS, h->order);
+    hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej,
+                       h->status, B, mode, h->order_tmp + VA_ORDER_BUCKETS, out);
     VA_HIP(hipGetLastError());
     return VA_OK;
+}
+
+va_status order_longest_first(va_handle *h, hipStream_t s) { return order_by_length(h, 0, s); }
+
+// The forward order is used when instances are refilled (more instances than
+// are in flight) and the caller did not ask for index order.
+bool reuse_order(const va_handle *h, int64_t lanes)
+{
+    return !(h->pb.flags & VA_FLAG_COLD_ORDER) && h->pb.batch > lanes;
 }
 
 // ---------------- persistent form ----------------
@@ -298,8 +319,11 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     a.status = h->status;
     a.counter = h->counters + 0;
     a.max_steps_seen = h->counters + 2;
+    const bool reuse = reuse_order(h, h->lanes_fwd);
+    a.order = reuse ? h->order_fwd : nullptr;
     const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
     VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+    if (reuse) return order_by_length(h, 1, s);
     return VA_OK;
 }
 
@@ -467,7 +491,8 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(VA_ERR_INVALID,
                     "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
     va::KernelSet ks{};
-    if (pb->flags & ~VA_FLAG_GRAPH) return fail(VA_ERR_INVALID, "unknown va_problem.flags bits");
+    if (pb->flags & ~(VA_FLAG_GRAPH | VA_FLAG_COLD_ORDER))
+        return fail(VA_ERR_INVALID, "unknown va_problem.flags bits");
     if (pb->n_obs < 0) return fail(VA_ERR_INVALID, "n_obs must be >= 0");
     if (pb->checkpoint_slots < 0 || pb->checkpoint_slots == 1 || pb->checkpoint_slots % 2 != 0)
         return fail(VA_ERR_INVALID, "checkpoint_slots must be 0 or an even number >= 2");
@@ -589,6 +614,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
     alloc((void **)&h->status, sizeof(int32_t) * B);
     alloc((void **)&h->counters, sizeof(int32_t) * 4);
     alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
+    alloc((void **)&h->order_fwd, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
     if (pb->n_obs > 0 || h->obs_ckpt) {
         alloc((void **)&h->obs_row, sizeof(int32_t) * std::max<size_t>(1, (size_t)B * pb->n_obs));
@@ -622,6 +648,18 @@ va_status va_create(const va_problem *pb, va_handle **out)
         return fail(e == hipErrorOutOfMemory ? VA_ERR_OUT_OF_MEMORY : VA_ERR_HIP,
                     "allocating the trajectory store (%zu rows x %lld instances): %s", rows,
                     (long long)B, hipGetErrorString(e));
+    }
+    {  // the first forward takes instances in index order
+        std::vector<int32_t> iota((size_t)B);
+        for (int64_t b = 0; b < B; ++b) iota[(size_t)b] = (int32_t)b;
+        e = hipMemcpy(h->order_fwd, iota.data(), sizeof(int32_t) * (size_t)B,
+                      hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            release(h);
+            delete h;
+            (void)hipGetLastError();
+            return fail(VA_ERR_HIP, "initialising the instance order: %s", hipGetErrorString(e));
+        }
     }
     *out = h;
     return VA_OK;
@@ -833,6 +871,8 @@ static va_status observed_enqueue(va_handle *h, const double *y0, const double *
     f.status = h->status;
     f.counter = h->counters + 0;
     f.max_steps_seen = h->counters + 2;
+    const bool reuse = reuse_order(h, h->lanes_obs_fwd);
+    f.order = reuse ? h->order_fwd : nullptr;
     f.obs_t = obs_t;
     f.obs_y = obs_y;
     f.K = h->pb.n_obs;
@@ -841,6 +881,7 @@ static va_status observed_enqueue(va_handle *h, const double *y0, const double *
     VA_HIP(h->ks.obs_fwd(f, blocks_for(std::min<int64_t>(h->lanes_obs_fwd, B), h->block),
                          h->block, s));
     h->have_forward = !h->obs_ckpt;
+    if (reuse && (st = order_by_length(h, 1, s))) return st;
     if ((st = order_longest_first(h, s))) return st;
     va::ObsRevArgs r;
     r.sp = f.sp;

@@ -119,81 +119,152 @@ struct Fehlberg78 {
 };
 
 // ---------------------------------------------------------------------------
-// Step-controller pow (x >= 0).  Own implementation from textbook methods:
-//   log x = e ln2 + 2 atanh(s), s = (m-1)/(m+1), m in [sqrt(1/2), sqrt(2)),
-//   exp t = 2^k exp(r), r = t - k ln2 (two-constant Cody-Waite split), Taylor to
-//   degree 13.  Only IEEE basic operations, fma, frexp, ldexp and rint, so the
-//   result is the same bits on the GPU and on the host.
+// Step-controller root x^(-1/q), q = 2..8 (the adjuster raises the error ratio
+// to -1/q).  Own method:
+//   x = m 2^e, m in [1,2), e = q k + j with 0 <= j < q:  x^(-1/q) = 2^-k (m 2^j)^(-1/q);
+//   start  y = P_q(m - 1.5) 2^(-j/q), P_q the degree-5 Chebyshev interpolant of
+//          m^(-1/q) on [1,2] (relative error <= 1.4e-5);
+//   refine two Newton steps for y^-q = m 2^j: y += (y/q)(1 - m 2^j y^q);
+//   scale  by 2^-k, exactly.
+// IEEE basic operations, fma and integer operations on the representation
+// only, so the GPU and the host obtain the same bits.  The dependent chain is
+// about 20 FP64 operations deep, against about 60 for exp(y log x) with an
+// IEEE division inside the log.  Within one ulp of the exact root.
 // ---------------------------------------------------------------------------
-#define VA_LN2_HI 0x1.62e42fee00000p-1
-#define VA_LN2_LO 0x1.a39ef35793c76p-33
-#define VA_INV_LN2 0x1.71547652b82fep+0
-#define VA_SQRT_HALF 0x1.6a09e667f3bcdp-1
+template <int Q>
+struct RootCoef;
+template <>
+struct RootCoef<2> {
+    static constexpr double p[6] = {0x1.a20d0b2f16c1fp-1, -0x1.16b40de0b165ap-2,
+                                    0x1.1559ba6062540p-3, -0x1.33fdf4c87b433p-4,
+                                    0x1.a0f6e83f40934p-5, -0x1.f69d70fc899e8p-6};
+    static constexpr double s[2] = {0x1p+0, 0x1.6a09e667f3bcdp-1};
+};
+template <>
+struct RootCoef<3> {
+    static constexpr double p[6] = {0x1.bf46914f5b764p-1, -0x1.8d95060abdbffp-3,
+                                    0x1.5ffe431f2a20ep-4, -0x1.6cc93eb91b616p-5,
+                                    0x1.d18071c6f18d9p-6, -0x1.0e87006cbb389p-6};
+    static constexpr double s[3] = {0x1p+0, 0x1.965fea53d6e3dp-1, 0x1.428a2f98d728bp-1};
+};
+template <>
+struct RootCoef<4> {
+    static constexpr double p[6] = {0x1.cea555e5f371ap-1, -0x1.346eefc659ea0p-3,
+                                    0x1.00192f0f33fe4p-4, -0x1.ffd60027966f0p-6,
+                                    0x1.3cd1d65f2357fp-6, -0x1.6962c4875515fp-7};
+    static constexpr double s[4] = {0x1p+0, 0x1.ae89f995ad3adp-1, 0x1.6a09e667f3bcdp-1,
+                                    0x1.306fe0a31b715p-1};
+};
+template <>
+struct RootCoef<5> {
+    static constexpr double p[6] = {0x1.d81ed544cf8d8p-1, -0x1.f79979e0f7a07p-4,
+                                    0x1.918282e8ac699p-5, -0x1.884dc61ebd241p-6,
+                                    0x1.dcc723482118fp-7, -0x1.0cd24fbb9fc7fp-7};
+    static constexpr double s[5] = {0x1p+0, 0x1.bdb8cdadbe120p-1, 0x1.8406003b2ae5cp-1,
+                                    0x1.51cb453b9536cp-1, 0x1.2611186bae675p-1};
+};
+template <>
+struct RootCoef<6> {
+    static constexpr double p[6] = {0x1.de8b600813da4p-1, -0x1.a9605d403f50fp-4,
+                                    0x1.49c4eebfec5aap-5, -0x1.3d52ab9af5948p-6,
+                                    0x1.7ce3ef31baabcp-7, -0x1.aa352530fcd10p-8};
+    static constexpr double s[6] = {0x1p+0, 0x1.c823e074ec129p-1, 0x1.965fea53d6e3dp-1,
+                                    0x1.6a09e667f3bcdp-1, 0x1.428a2f98d728bp-1,
+                                    0x1.1f59ac3c7d6c0p-1};
+};
+template <>
+struct RootCoef<7> {
+    static constexpr double p[6] = {0x1.e32fbc32c3ccbp-1, -0x1.702525e04166bp-4,
+                                    0x1.179a8cef4f954p-5, -0x1.0a17a5e275aafp-6,
+                                    0x1.3c8ea867dbd19p-7, -0x1.60416b7d9015bp-8};
+    static constexpr double s[7] = {0x1p+0, 0x1.cfbb031a741a5p-1, 0x1.a402feeb9c533p-1,
+                                    0x1.7c6a1f29e2ce6p-1, 0x1.588cea3f093bep-1,
+                                    0x1.381147622f886p-1, 0x1.1aa59c4115e7dp-1};
+};
+template <>
+struct RootCoef<8> {
+    static constexpr double p[6] = {0x1.e6b29102ed063p-1, -0x1.4477b7dbbc595p-4,
+                                    0x1.e5320a8f3a376p-6, -0x1.c9e63649fd2a0p-7,
+                                    0x1.0e8b165b61159p-7, -0x1.2bcadec4f337fp-8};
+    static constexpr double s[8] = {0x1p+0, 0x1.d5818dcfba487p-1, 0x1.ae89f995ad3adp-1,
+                                    0x1.8ace5422aa0dbp-1, 0x1.6a09e667f3bcdp-1,
+                                    0x1.4bfdad5362a27p-1, 0x1.306fe0a31b715p-1,
+                                    0x1.172b83c7d517bp-1};
+};
 
-VA_HD double log_pos(double x)
+// y^Q by a fixed sequence of products
+template <int Q>
+VA_HD double root_ipow(double y)
 {
-    int e;
-    double m = frexp(x, &e);
-    if (m < VA_SQRT_HALF) {
-        m = m * 2.0;
-        e = e - 1;
+    const double y2 = y * y;
+    if constexpr (Q == 2) return y2;
+    else if constexpr (Q == 3) return y2 * y;
+    else if constexpr (Q == 4) return y2 * y2;
+    else if constexpr (Q == 5) return (y2 * y2) * y;
+    else if constexpr (Q == 6) { const double y3 = y2 * y; return y3 * y3; }
+    else if constexpr (Q == 7) { const double y3 = y2 * y, y4 = y2 * y2; return y4 * y3; }
+    else { const double y4 = y2 * y2; return y4 * y4; }
+}
+
+// 2^(-j/Q) for 0 <= j < Q by selects (no indexed load)
+template <int Q>
+VA_HD double root_scale(int j)
+{
+    double s = RootCoef<Q>::s[0];
+#pragma unroll
+    for (int i = 1; i < Q; ++i) s = (j == i) ? RootCoef<Q>::s[i] : s;
+    return s;
+}
+
+VA_HD double bits_double(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+// P_Q(u) by Estrin's scheme
+template <int Q>
+VA_HD double root_poly(double u, double u2, double u4)
+{
+    using R = RootCoef<Q>;
+    const double p01 = fma(R::p[1], u, R::p[0]);
+    const double p23 = fma(R::p[3], u, R::p[2]);
+    const double p45 = fma(R::p[5], u, R::p[4]);
+    return fma(p45, u4, fma(p23, u2, p01));
+}
+
+// x^(-1/QA) if use_a, else x^(-1/QB).  Both candidates' integer splits,
+// polynomials and scale constants are formed unconditionally and the lane's
+// one is selected (no branch, no table load), so lanes that pick different
+// orders execute one instruction stream.
+template <int QA, int QB>
+VA_HD double ctrl_root(double x, bool use_a)
+{
+    static_assert(QA >= 2 && QA <= 8 && QB >= 2 && QB <= 8, "root order out of range");
+    const bool sub = x < 2.2250738585072014e-308;  // subnormal (or <= 0: replaced below)
+    const double xs = sub ? x * 0x1p64 : x;
+    const uint64_t bits = __builtin_bit_cast(uint64_t, xs);
+    const int e = (int)(bits >> 52) - 1023 - (sub ? 64 : 0);
+    const uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    const int ea = e + QA * 2048, eb = e + QB * 2048;  // > 0 for every finite x > 0
+    const int ka = ea / QA - 2048, kb = eb / QB - 2048;
+    const int ja = ea - (ea / QA) * QA, jb = eb - (eb / QB) * QB;
+    const int k = use_a ? ka : kb;
+    const int j = use_a ? ja : jb;
+    const double m = bits_double(mb);
+    const double xr = bits_double(mb + ((uint64_t)(uint32_t)j << 52));  // m 2^j, exact
+    const double u = m - 1.5, u2 = u * u, u4 = u2 * u2;
+    const double ya = root_poly<QA>(u, u2, u4) * root_scale<QA>(ja);
+    const double yb = root_poly<QB>(u, u2, u4) * root_scale<QB>(jb);
+    double y = use_a ? ya : yb;
+    const double iq = use_a ? 1.0 / (double)QA : 1.0 / (double)QB;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double yqa = root_ipow<QA>(y), yqb = root_ipow<QB>(y);
+        const double r = fma(-xr, use_a ? yqa : yqb, 1.0);
+        y = fma(y * iq, r, y);
     }
-    double s = (m - 1.0) / (m + 1.0);
-    double z = s * s;
-    // q = 1/3 + z/5 + ... + z^10/23 by Estrin's scheme (short dependency chains:
-    // a dependent FP64 FMA costs ~32 cycles on gfx950)
-    double z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
-    double p01 = fma(1.0 / 5.0, z, 1.0 / 3.0);
-    double p23 = fma(1.0 / 9.0, z, 1.0 / 7.0);
-    double p45 = fma(1.0 / 13.0, z, 1.0 / 11.0);
-    double p67 = fma(1.0 / 17.0, z, 1.0 / 15.0);
-    double p89 = fma(1.0 / 21.0, z, 1.0 / 19.0);
-    double p03 = fma(p23, z2, p01);
-    double p47 = fma(p67, z2, p45);
-    double p8a = fma(1.0 / 23.0, z2, p89);
-    double p07 = fma(p47, z4, p03);
-    double q = fma(p8a, z8, p07);
-    double zq = z * q;
-    double s2 = s + s;
-    double lm = fma(s2, zq, s2);
-    double de = (double)e;
-    return fma(de, VA_LN2_HI, fma(de, VA_LN2_LO, lm));
-}
-
-VA_HD double exp_(double t)
-{
-    if (t != t) return t;
-    if (t > 709.8) return __builtin_huge_val();
-    if (t < -745.2) return 0.0;
-    double k = rint(t * VA_INV_LN2);
-    double r = fma(-k, VA_LN2_HI, t);
-    r = fma(-k, VA_LN2_LO, r);
-    // sum_{i=0}^{13} r^i / i! by Estrin's scheme
-    double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
-    double e01 = fma(1.0, r, 1.0);
-    double e23 = fma(1.0 / 6.0, r, 0.5);
-    double e45 = fma(1.0 / 120.0, r, 1.0 / 24.0);
-    double e67 = fma(1.0 / 5040.0, r, 1.0 / 720.0);
-    double e89 = fma(1.0 / 362880.0, r, 1.0 / 40320.0);
-    double eab = fma(1.0 / 39916800.0, r, 1.0 / 3628800.0);
-    double ecd = fma(1.0 / 6227020800.0, r, 1.0 / 479001600.0);
-    double e03 = fma(e23, r2, e01);
-    double e47 = fma(e67, r2, e45);
-    double e8b = fma(eab, r2, e89);
-    double e07 = fma(e47, r4, e03);
-    double e8d = fma(ecd, r4, e8b);
-    double q = fma(e8d, r8, e07);
-    return ldexp(q, (int)k);
-}
-
-VA_HD double ctrl_pow(double x, double y)
-{
-    if (x != x || y != y) return x + y;
-    if (y == 0.0) return 1.0;
-    if (x == 0.0) return y < 0.0 ? __builtin_huge_val() : 0.0;
-    if (x < 0.0) return __builtin_nan("");
-    if (x == __builtin_huge_val()) return y < 0.0 ? 0.0 : __builtin_huge_val();
-    return exp_(y * log_pos(x));
+    double res = y * bits_double((uint64_t)(uint32_t)(1023 - k) << 52);
+    res = (x == __builtin_huge_val()) ? 0.0 : res;
+    res = (x == 0.0) ? __builtin_huge_val() : res;
+    res = (x < 0.0) ? __builtin_nan("") : res;
+    return (x != x) ? x : res;
 }
 
 // ---------------------------------------------------------------------------
@@ -222,39 +293,23 @@ struct FivePow {
     }
 };
 
-template <class Tab>
-VA_HD double decrease_step(double h, double err)
-{
-    double f = 0.9 * ctrl_pow(err, -1.0 / (double)(Tab::ERR_ORDER - 1));
-    f = (f < 0.2) ? 0.2 : f;
-    return h * f;
-}
-
-template <class Tab>
-VA_HD double increase_step(double h, double err)
-{
-    if (err < 0.5) {
-        constexpr double lo = FivePow<Tab::ORDER>::inv();
-        err = (lo < err) ? err : lo;
-        h = h * (0.9 * ctrl_pow(err, -1.0 / (double)Tab::ORDER));
-    }
-    return h;
-}
-
-// Accept/reject decision and the next step size with ONE pow evaluation:
-// decrease_step and increase_step above, with the pow argument and exponent
-// selected first.  Lanes of a wave that take different decisions then share a
-// single pow instead of executing one per branch; every value is computed with
-// the same operations as in the two functions (bit-identical).
+// Accept/reject decision and the next step size (default_step_adjuster with
+// max_dt = 0):
+//   reject (err > 1):   h * max(0.9 err^(-1/(ERR_ORDER-1)), 0.2)
+//   accept, err < 0.5:  h * 0.9 max(5^-ORDER, err)^(-1/ORDER)
+//   accept otherwise:   h
+// with ONE root evaluation: its argument and order are selected first, so lanes
+// of a wave that take different decisions share one instruction stream.
 template <class Tab>
 VA_HD bool control_step(double h, double err, double &h_next)
 {
     constexpr double lo = FivePow<Tab::ORDER>::inv();
-    constexpr double e_dec = -1.0 / (double)(Tab::ERR_ORDER - 1);
-    constexpr double e_inc = -1.0 / (double)Tab::ORDER;
+    // fixed-step tableaus (no error estimate) never reach the controller
+    constexpr int QI = Tab::ORDER >= 2 ? Tab::ORDER : 2;
+    constexpr int QD = Tab::ERR_ORDER >= 3 ? Tab::ERR_ORDER - 1 : 2;
     const bool accept = !(err > 1.0);
     const double base = accept ? ((lo < err) ? err : lo) : err;
-    const double f = 0.9 * ctrl_pow(base, accept ? e_inc : e_dec);
+    const double f = 0.9 * ctrl_root<QI, QD>(base, accept);
     if (!accept) {
         h_next = h * ((f < 0.2) ? 0.2 : f);
     } else {

@@ -188,6 +188,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
             b = gidx[wv][grp];
             __builtin_amdgcn_wave_barrier();
             if (b >= B) break;
+            if (a.order) b = a.order[b];
             r = a.p[(int64_t)i * B + b];
 #pragma unroll
             for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];

@@ -23,6 +23,7 @@ struct FwdArgs {
     int32_t *n_acc, *n_rej, *status;
     int32_t *counter;
     int32_t *max_steps_seen;
+    const int32_t *order = nullptr;  // [B] or null: the k-th instance taken is order[k]
 };
 
 // Persistent reverse over the stored trajectory.
@@ -96,6 +97,7 @@ struct ObsFwdArgs {
     int32_t *n_acc, *n_rej, *status;
     int32_t *counter;
     int32_t *max_steps_seen;
+    const int32_t *order = nullptr;  // [B] or null: the k-th instance taken is order[k]
     const double *obs_t;   // [K]
     const double *obs_y;   // [K][N]
     int32_t K;

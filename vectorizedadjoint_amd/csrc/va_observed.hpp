@@ -67,6 +67,7 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
         if (need) {
             b = atomicAdd(a.counter, 1);
             if (b >= B) break;
+            if (a.order) b = a.order[b];
             Sys::load(q, a.p, B, b, sp.kconst);
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = a.y[(int64_t)i * B + b];
