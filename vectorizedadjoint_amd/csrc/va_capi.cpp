@@ -121,11 +121,17 @@ __global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const 
 }
 
 // cursor[k] = number of instances in buckets above k (longest bucket first).
-__global__ void k_order_offsets(const int32_t *hist, int32_t *cursor)
+// With n_long: the instances in buckets >= VA_LONG_BUCKET, at most VA_LONG_MAX
+// (the leading positions of the order that the long-instance kernel takes).
+constexpr int VA_LONG_BUCKET = 16;  // 256 attempts and more
+constexpr int VA_LONG_MAX = 256;    // one wave each
+
+__global__ void k_order_offsets(const int32_t *hist, int32_t *cursor, int32_t *n_long)
 {
     if (threadIdx.x == 0) {
         int32_t acc = 0;
         for (int k = VA_ORDER_BUCKETS - 1; k >= 0; --k) {
+            if (k == VA_LONG_BUCKET - 1 && n_long) *n_long = acc < VA_LONG_MAX ? acc : VA_LONG_MAX;
             cursor[k] = acc;
             acc += hist[k];
         }
@@ -172,7 +178,9 @@ struct va_handle {
     int32_t *pl_done = nullptr;
     // bookkeeping [B] and counters
     int32_t *n_acc = nullptr, *n_rej = nullptr, *status = nullptr;
-    int32_t *counters = nullptr;  // [0] fwd queue, [1] rev queue, [2] max steps, [3] active
+    // [0] fwd queue, [1] rev queue, [2] max steps, [3] active, [5] long instances
+    // at the head of order_fwd (from the previous forward's attempts)
+    int32_t *counters = nullptr;
     int32_t *order = nullptr;     // [B] reverse order, longest first
     int32_t *order_fwd = nullptr; // [B] next forward's order (previous attempts, longest first)
     int32_t *order_tmp = nullptr; // [2 * VA_ORDER_BUCKETS] histogram, cursors
@@ -191,6 +199,9 @@ struct va_handle {
     int32_t graph_cost = 0;
     int32_t *host_counter = nullptr;
     bool have_forward = false;
+    // va_solve with the long instances on a second stream (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -272,6 +283,9 @@ void release(va_handle *h)
     hipFree(h->abar_part);
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->cap_stream) hipStreamDestroy(h->cap_stream);
+    if (h->side) hipStreamDestroy(h->side);
+    if (h->ev_fork) hipEventDestroy(h->ev_fork);
+    if (h->ev_join) hipEventDestroy(h->ev_join);
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
@@ -287,7 +301,7 @@ va_status order_by_length(va_handle *h, int mode, hipStream_t s)
     hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej, h->status,
                        B, mode, h->order_tmp);
     hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
-                       h->order_tmp + VA_ORDER_BUCKETS);
+                       h->order_tmp + VA_ORDER_BUCKETS, mode == 1 ? h->counters + 5 : nullptr);
     hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej,
                        h->status, B, mode, h->order_tmp + VA_ORDER_BUCKETS, out);
     VA_HIP(hipGetLastError());
@@ -353,6 +367,68 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
     VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     return VA_OK;
+}
+
+// va_solve with the instances that ran long in the previous solve (the head of
+// order_fwd, counted in counters[5]) on a second stream, one wave each (their
+// forwards, then their reverses), while the persistent kernels run the forward
+// and then the reverse of the rest of the batch on the caller's stream.  The
+// persistent kernels leave room on the device for the long-instance waves.
+bool split_solve(const va_handle *h)
+{
+    return h->side && reuse_order(h, h->lanes_fwd);
+}
+
+va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const double *cost_w,
+                      double *J, double *dJdy0, double *dJdp, hipStream_t s)
+{
+    const int64_t B = h->pb.batch;
+    va_status st = fill_i32(h->counters, 0, 4, s);
+    if (st) return st;
+    va::FwdArgs fa;
+    fa.sp = solve_params(h);
+    fa.y = yT;
+    fa.p = p;
+    fa.traj = h->traj;
+    fa.n_acc = h->n_acc;
+    fa.n_rej = h->n_rej;
+    fa.status = h->status;
+    fa.counter = h->counters + 0;
+    fa.max_steps_seen = h->counters + 2;
+    fa.order = h->order_fwd;
+    fa.split = h->counters + 5;
+    fa.part = 1;
+    va::RevArgs ra;
+    ra.sp = fa.sp;
+    ra.p = p;
+    ra.traj = h->traj;
+    ra.n_acc = h->n_acc;
+    ra.status = h->status;
+    ra.cost = cost;
+    ra.cost_w = cost_w;
+    ra.yT = yT;
+    ra.J = J;
+    ra.lambda = dJdy0;
+    ra.mu = dJdp;
+    ra.nout = 1;
+    ra.counter = h->counters + 1;
+    ra.order = h->order_fwd;
+    ra.split = h->counters + 5;
+    ra.part = 1;
+    VA_HIP(hipEventRecord(h->ev_fork, s));
+    VA_HIP(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    VA_HIP(h->ks.long_fwd(fa, VA_LONG_MAX / 4, 256, h->side));
+    VA_HIP(h->ks.long_rev(ra, VA_LONG_MAX / 4, 256, h->side));
+    VA_HIP(hipEventRecord(h->ev_join, h->side));
+    // each persistent kernel leaves the room of VA_LONG_MAX waves of its own size
+    const int per_block = h->ks.slots(h->block);
+    const int reserve = blocks_for((int64_t)VA_LONG_MAX * h->ks.slots(64), per_block);
+    const int gf = blocks_for(std::min<int64_t>(h->lanes_fwd, B), per_block);
+    const int gr = blocks_for(std::min<int64_t>(h->lanes_rev, B), per_block);
+    VA_HIP(h->ks.fwd(fa, std::max(1, gf - reserve), h->block, s));
+    VA_HIP(h->ks.rev(ra, std::max(1, gr - reserve), h->block, s));
+    VA_HIP(hipStreamWaitEvent(s, h->ev_join, 0));
+    return order_by_length(h, 1, s);
 }
 
 // ---------------- plain form ----------------
@@ -612,7 +688,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
     alloc((void **)&h->n_acc, sizeof(int32_t) * B);
     alloc((void **)&h->n_rej, sizeof(int32_t) * B);
     alloc((void **)&h->status, sizeof(int32_t) * B);
-    alloc((void **)&h->counters, sizeof(int32_t) * 4);
+    alloc((void **)&h->counters, sizeof(int32_t) * 8);
     alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_fwd, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
@@ -649,16 +725,29 @@ va_status va_create(const va_problem *pb, va_handle **out)
                     "allocating the trajectory store (%zu rows x %lld instances): %s", rows,
                     (long long)B, hipGetErrorString(e));
     }
+    // the second stream and its events of split solves (created here: va_solve may
+    // run under stream capture)
+    if (e == hipSuccess && ks.long_fwd && pb->adaptive && pb->mode == VA_MODE_PERSISTENT) {
+        e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
+    }
+    // no previous forward: zero attempt counts and no long instances
+    if (e == hipSuccess) e = hipMemset(h->counters, 0, sizeof(int32_t) * 8);
+    if (e == hipSuccess) e = hipMemset(h->n_acc, 0, sizeof(int32_t) * B);
+    if (e == hipSuccess) e = hipMemset(h->n_rej, 0, sizeof(int32_t) * B);
     {  // the first forward takes instances in index order
         std::vector<int32_t> iota((size_t)B);
         for (int64_t b = 0; b < B; ++b) iota[(size_t)b] = (int32_t)b;
-        e = hipMemcpy(h->order_fwd, iota.data(), sizeof(int32_t) * (size_t)B,
-                      hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(h->order_fwd, iota.data(), sizeof(int32_t) * (size_t)B,
+                          hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             release(h);
             delete h;
             (void)hipGetLastError();
-            return fail(VA_ERR_HIP, "initialising the instance order: %s", hipGetErrorString(e));
+            return fail(VA_ERR_HIP, "initialising streams, step counts and instance order: %s",
+                        hipGetErrorString(e));
         }
     }
     *out = h;
@@ -825,6 +914,9 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
         int32_t smax = 0;
         if ((st = read_max_steps(h, &smax, s))) return st;
         if ((st = reverse_plain_seed(h, p, dJdy0, dJdp, smax, s))) return st;
+    } else if (split_solve(h)) {
+        if ((st = solve_split(h, yT, p, cost, cost_weights, J, dJdy0, dJdp, s))) return st;
+        h->have_forward = true;
     } else {
         if ((st = forward_persistent(h, yT, p, s))) return st;
         h->have_forward = true;

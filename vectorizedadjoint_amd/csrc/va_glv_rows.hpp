@@ -4,7 +4,9 @@
 // Layout.  A wave64 holds G = 64/N instance groups (N = 10: 6 groups, 60 busy
 // lanes).  Lane i of a group owns state component i, row i of A (for the
 // growth rates g_i = r_i + sum_j A_ij x_j), column i of A (for the adjoint
-// term sum_k u_k A_ki) and row i of dJ/dA.  Stage vectors are exchanged
+// term sum_k u_k A_ki) and, in the reverse sweep, column i of dJ/dA (sum of
+// u_k X_i: the u-vector is gathered for the adjoint term anyway, X_i is the
+// lane's own).  Stage vectors are exchanged
 // through a per-wave LDS slab: every lane writes its component, the group
 // reads the whole vector back as broadcast 16-byte loads.  Nothing crosses
 // waves, so there is no block barrier; LDS operations of one wave complete in
@@ -146,11 +148,30 @@ __device__ __forceinline__ double adjoint_col(const double (&Acol)[NN], double w
     return ae + ao;
 }
 
+// Where a group's instances come from: the persistent kernels take queue
+// positions [lo, hi) from an atomic counter (mapped through `order` when given;
+// `split`/`part` select the positions below or from *split); OneShot bodies
+// run the single instance `one` on group 0 of the wave.
+struct QueueRange {
+    int64_t lo, hi;
+};
+
+__device__ __forceinline__ QueueRange queue_range(int64_t B, const int32_t *split, int part)
+{
+    QueueRange q{0, B};
+    if (split) {
+        const int64_t k = *split;
+        if (part == 0) q.hi = k < B ? k : B;
+        else q.lo = k < B ? k : B;
+    }
+    return q;
+}
+
 // ---------------------------------------------------------------------------
 // Forward sweep (persistent, refill per instance group)
 // ---------------------------------------------------------------------------
-template <int NN, class Tab, bool Adaptive>
-__global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
+template <int NN, class Tab, bool Adaptive, bool OneShot>
+__device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
 {
     using L = GlvRows<NN>;
     constexpr int F = L::F;
@@ -160,7 +181,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int grp = lane / NN, i = lane - grp * NN;
-    if (grp >= L::GPW) return;
+    if (OneShot ? grp != 0 : grp >= L::GPW) return;
     double *slab = reinterpret_cast<double *>(slab2[wv]);
     double *slabb = reinterpret_cast<double *>(slabb2[wv]);
     const int gbase = grp * NN;
@@ -176,19 +197,26 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
     double t = 0.0, dt = 0.0;
     int64_t b = 0;
     int32_t count = 0, rej = 0, tries = 0, st = LANE_OK;
-    bool new_step = true, have_k0 = false, need = true;
+    bool new_step = true, have_k0 = false, need = true, taken = false;
     double *rows = nullptr;
     double v[NN];
+    const QueueRange q = queue_range(B, a.split, a.part);
 
     for (;;) {
-        prioritise_long(!need && count + rej > VA_LONG_ATTEMPTS);
+        prioritise_long(OneShot || (!need && count + rej > VA_LONG_ATTEMPTS));
         if (need) {
-            if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
-            __builtin_amdgcn_wave_barrier();
-            b = gidx[wv][grp];
-            __builtin_amdgcn_wave_barrier();
-            if (b >= B) break;
-            if (a.order) b = a.order[b];
+            if constexpr (OneShot) {
+                if (taken) break;
+                taken = true;
+                b = one;
+            } else {
+                if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
+                __builtin_amdgcn_wave_barrier();
+                b = q.lo + gidx[wv][grp];
+                __builtin_amdgcn_wave_barrier();
+                if (b >= q.hi) break;
+                if (a.order) b = a.order[b];
+            }
             r = a.p[(int64_t)i * B + b];
 #pragma unroll
             for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
@@ -308,46 +336,75 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
     }
 }
 
+template <int NN, class Tab, bool Adaptive>
+__global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
+{
+    glv_rows_fwd_body<NN, Tab, Adaptive, false>(a, 0);
+}
+
 // ---------------------------------------------------------------------------
-// Reverse sweep (persistent, refill per instance group)
+// Reverse sweep (persistent, refill per instance group), software-pipelined.
+//
+// The reverse of step n needs the stage inputs X_m and growth rates g_m of
+// step n, recomputed from the stored (h_n, y_n); that recompute does not
+// depend on the adjoint.  So each sweep iteration recomputes step n-1 while it
+// reverses step n: stage i of the recompute runs beside backward stage S-1-i,
+// the two share one LDS exchange round (two writes, three broadcast gathers),
+// and their dependent chains interleave.  The recomputed X-vectors and the
+// lanes' g go to LDS (two buffers per wave, alternating steps), where the
+// backward of the next iteration reads them.  The first iteration of an
+// instance (or of a further seed) only recomputes.  Every value is formed by
+// the same operations in the same order as in the unpipelined sweep.
 // ---------------------------------------------------------------------------
-template <int NN, class Tab>
-__global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
+template <int NN, class Tab, bool OneShot>
+__device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
 {
     using L = GlvRows<NN>;
     constexpr int F = L::F, S = Tab::S, P = NN + NN * NN;
-    // per wave: S stage-input vectors + one scratch vector, 64 doubles each
-    __shared__ double2 xsl2[L::WAVES][S][32];
+    // per wave: two step buffers of S stage-input vectors and S growth-rate
+    // vectors (64 doubles each), one vector for the adjoint products u
+    __shared__ double2 xsl2[L::WAVES][2][S][32];
+    __shared__ double gsl[L::WAVES][2][S][64];
     __shared__ double2 usl2[L::WAVES][32];
     __shared__ int gidx[L::WAVES][L::GPW];
 
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int grp = lane / NN, i = lane - grp * NN;
-    if (grp >= L::GPW) return;
+    if (OneShot ? grp != 0 : grp >= L::GPW) return;
     const int gbase = grp * NN;
     double *uslab = reinterpret_cast<double *>(usl2[wv]);
     const SolveParams sp = a.sp;
     const int64_t B = sp.B;
     const double qnan = __builtin_nan("");
 
+    // Abar: column i of dJ/dA (entries A_ki, k = 0..N-1)
     double Arow[NN], Acol[NN], Abar[NN], r = 0.0, rbar = 0.0, lam = 0.0;
-    // row n's (h_n, y_n) of this lane, loaded one step ahead of its use
-    double h_cur = 0.0, y_cur = 0.0;
+    double Kn[S], Kb[S];
+    // row nr's (h, y_i) of this lane, loaded one iteration ahead of its use
+    double h_nxt = 0.0, y_nxt = 0.0, h_back = 0.0;
     int64_t b = 0;
-    int32_t o = 0, n = -1;
-    bool need = true;
+    int32_t o = 0, nr = -1;
+    int par = 0;  // LDS buffer holding the step being reversed
+    bool need = true, back = false, taken = false;
     const double *rows = nullptr;
     double v[NN];
+    const QueueRange q = queue_range(B, a.split, a.part);
 
     for (;;) {
-        prioritise_long(!need && n > VA_LONG_ATTEMPTS);
+        prioritise_long(OneShot || (!need && nr > VA_LONG_ATTEMPTS));
         if (need) {
-            if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
-            __builtin_amdgcn_wave_barrier();
-            b = gidx[wv][grp];
-            __builtin_amdgcn_wave_barrier();
-            if (b >= B) break;
-            if (a.order) b = a.order[b];
+            if constexpr (OneShot) {
+                if (taken) break;
+                taken = true;
+                b = one;
+            } else {
+                if (i == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
+                __builtin_amdgcn_wave_barrier();
+                b = q.lo + gidx[wv][grp];
+                __builtin_amdgcn_wave_barrier();
+                if (b >= q.hi) break;
+                if (a.order) b = a.order[b];
+            }
             if (a.status[b] != LANE_OK) {
                 const int nseed = a.cost ? 1 : a.nout;
                 for (int oo = 0; oo < nseed; ++oo) {
@@ -388,73 +445,99 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
             rbar = 0.0;
 #pragma unroll
             for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
-            n = a.n_acc[b] - 1;
-            if (n >= 0) {
-                h_cur = rows[(size_t)n * F + 1];
-                y_cur = rows[(size_t)n * F + 2 + i];
+            nr = a.n_acc[b] - 1;
+            back = false;
+            if (nr >= 0) {
+                h_nxt = rows[(size_t)nr * F + 1];
+                y_nxt = rows[(size_t)nr * F + 2 + i];
             }
             need = false;
             wait_vmem();
         }
-        if (n >= 0) {
-            const double h = h_cur;
-            const double yn = y_cur;
-            if (n > 0) {
-                const double *prev = rows + (size_t)(n - 1) * F;
-                h_cur = prev[1];
-                y_cur = prev[2 + i];
+        if (nr >= 0 || back) {
+            // recompute step nr (stage inputs from (h_nr, y_nr), backpropagation.hpp:24-64)
+            // beside the reverse of step nr + 1 (:104-153)
+            const double hr = h_nxt, yr = y_nxt;
+            if (nr >= 1) {
+                const double *prev = rows + (size_t)(nr - 1) * F;
+                h_nxt = prev[1];
+                y_nxt = prev[2 + i];
             }
-            double X[S], g[S], K[S], Kb[S];
-            // recompute the stages from y_n (backpropagation.hpp:24-64)
+            const int pr = par ^ 1;
+            double *gr = &gsl[wv][pr][0][0];
+            if (!back) {
 #pragma unroll
-            for (int m = 0; m < S; ++m) {
-                double x = yn;
+                for (int m = 0; m < S; ++m) {
+                    double x = yr;
 #pragma unroll
-                for (int j = 0; j < S; ++j)
-                    if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * h, K[j], x);
-                X[m] = x;
-                double *xs = reinterpret_cast<double *>(xsl2[wv][m]);
-                exchange<NN>(xs, lane, gbase, x, v);
-                g[m] = growth_row<NN>(Arow, r, v);
-                K[m] = x * g[m];
+                    for (int j = 0; j < S; ++j)
+                        if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * hr, Kn[j], x);
+                    double *xs = reinterpret_cast<double *>(xsl2[wv][pr][m]);
+                    exchange<NN>(xs, lane, gbase, x, v);
+                    const double g = growth_row<NN>(Arow, r, v);
+                    gr[m * 64 + lane] = g;
+                    Kn[m] = x * g;
+                }
+            } else {
+                const double *gb = &gsl[wv][par][0][0];
+#pragma unroll
+                for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h_back) * lam;
+                double ybar = lam;
+#pragma unroll
+                for (int ii = 0; ii < S; ++ii) {
+                    const int m = S - 1 - ii;
+                    const double *xb = reinterpret_cast<const double *>(xsl2[wv][par][m]);
+                    const double w = Kb[m];
+                    const double xown = xb[lane];
+                    const double u = w * xown;
+                    const double wg = w * gb[m * 64 + lane];
+                    double x = yr;
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        if (j < ii && Tab::A[ii][j] != 0.0) x = fma(Tab::A[ii][j] * hr, Kn[j], x);
+                    double *xs = reinterpret_cast<double *>(xsl2[wv][pr][ii]);
+                    double vu[NN];
+                    xs[lane] = x;
+                    uslab[lane] = u;
+                    __builtin_amdgcn_wave_barrier();
+                    gather_vec<NN>(xs, gbase, v);
+                    gather_vec<NN>(uslab, gbase, vu);
+                    __builtin_amdgcn_wave_barrier();
+                    const double g = growth_row<NN>(Arow, r, v);
+                    gr[ii * 64 + lane] = g;
+                    Kn[ii] = x * g;
+                    const double xi = adjoint_col<NN>(Acol, wg, vu);
+                    rbar = rbar + u;
+                    // column i of dJ/dA: dA_ki += u_k X_i, from the gathered u-vector and
+                    // the lane's own X (no X-vector gather)
+#pragma unroll
+                    for (int k = 0; k < NN; ++k) Abar[k] = Abar[k] + vu[k] * xown;
+                    ybar = ybar + xi;
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h_back, xi, Kb[j]);
+                }
+                lam = ybar;
             }
-            // seed (:104-112)
-#pragma unroll
-            for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h) * lam;
-            double ybar = lam;
-            // stages s..1 (:126-153)
-#pragma unroll
-            for (int m = S - 1; m >= 0; --m) {
-                const double w = Kb[m];
-                const double u = w * X[m];
-                exchange<NN>(uslab, lane, gbase, u, v);
-                const double xi = adjoint_col<NN>(Acol, w * g[m], v);
-                rbar = rbar + u;
-                gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
-#pragma unroll
-                for (int j = 0; j < NN; ++j) Abar[j] = Abar[j] + u * v[j];
-                ybar = ybar + xi;
-#pragma unroll
-                for (int j = 0; j < S; ++j)
-                    if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h, xi, Kb[j]);
-            }
-            lam = ybar;
-            --n;
+            h_back = hr;
+            par = pr;
+            back = nr >= 0;
+            --nr;
         }
-        if (n < 0) {
+        if (!back) {
             const int64_t lo = (int64_t)o * NN, po = (int64_t)o * P;
             a.lambda[(lo + i) * B + b] = lam;
             if (a.cost) {
                 a.mu[(po + i) * B + b] = 0.0 + rbar;
 #pragma unroll
-                for (int j = 0; j < NN; ++j) a.mu[(po + NN * (i + 1) + j) * B + b] = 0.0 + Abar[j];
+                for (int k = 0; k < NN; ++k) a.mu[(po + NN * (k + 1) + i) * B + b] = 0.0 + Abar[k];
             } else {
                 double *m0 = a.mu + (po + i) * B + b;
                 *m0 = *m0 + rbar;
 #pragma unroll
-                for (int j = 0; j < NN; ++j) {
-                    double *mj = a.mu + (po + NN * (i + 1) + j) * B + b;
-                    *mj = *mj + Abar[j];
+                for (int k = 0; k < NN; ++k) {
+                    double *mk = a.mu + (po + NN * (k + 1) + i) * B + b;
+                    *mk = *mk + Abar[k];
                 }
             }
             ++o;
@@ -465,15 +548,46 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
                 rbar = 0.0;
 #pragma unroll
                 for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
-                n = a.n_acc[b] - 1;
-                if (n >= 0) {
-                    h_cur = rows[(size_t)n * F + 1];
-                    y_cur = rows[(size_t)n * F + 2 + i];
+                nr = a.n_acc[b] - 1;
+                if (nr >= 0) {
+                    h_nxt = rows[(size_t)nr * F + 1];
+                    y_nxt = rows[(size_t)nr * F + 2 + i];
                 }
                 wait_vmem();
             }
         }
     }
+}
+
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
+{
+    glv_rows_rev_body<NN, Tab, false>(a, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Instances expected to run long: queue positions [0, *split) of the forward
+// order, one instance per wave (group 0), in two launches on a stream of their
+// own -- all forwards, then all reverses -- while the persistent kernels run the
+// rest of the batch (positions from *split) on the caller's stream.  A wave
+// never waits for another instance, and the long chains start at once instead
+// of wherever the queue would reach them.  (Two launches rather than one fused
+// kernel: each fits beside the persistent kernel of its phase on a CU.)
+// ---------------------------------------------------------------------------
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs a)
+{
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= *a.split || w >= a.sp.B) return;
+    glv_rows_fwd_body<NN, Tab, true, true>(a, a.order[w]);
+}
+
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_long_rev(RevArgs a)
+{
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= *a.split || w >= a.sp.B) return;
+    glv_rows_rev_body<NN, Tab, true>(a, a.order[w]);
 }
 
 // ---------------------------------------------------------------------------
@@ -693,6 +807,16 @@ struct LaunchGlvRows {
         hipLaunchKernelGGL((k_glv_rows_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
+    static hipError_t lfwd(const FwdArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_long_fwd<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static hipError_t lrev(const RevArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_long_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
     static hipError_t pfwd(const PlainFwdArgs &a, int grid, int block, hipStream_t s)
     {
         hipLaunchKernelGGL((k_glv_rows_plain_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s,
@@ -732,6 +856,10 @@ struct LaunchGlvRows {
         k->occ_rev = &occ_rev;
         k->slots = &slots;
         k->layout = 1;
+        if constexpr (Ad) {
+            k->long_fwd = &lfwd;
+            k->long_rev = &lrev;
+        }
     }
 };
 

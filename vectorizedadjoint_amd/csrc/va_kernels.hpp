@@ -24,6 +24,8 @@ struct FwdArgs {
     int32_t *counter;
     int32_t *max_steps_seen;
     const int32_t *order = nullptr;  // [B] or null: the k-th instance taken is order[k]
+    const int32_t *split = nullptr;  // null: queue positions [0, B); else part 0 takes
+    int32_t part = 0;                // [0, *split), part 1 takes [*split, B)
 };
 
 // Persistent reverse over the stored trajectory.
@@ -44,6 +46,8 @@ struct RevArgs {
     int32_t nout;
     int32_t *counter;
     const int32_t *order;  // [B] or null: the k-th instance taken is order[k]
+    const int32_t *split = nullptr;  // as in FwdArgs
+    int32_t part = 0;
 };
 
 // Plain form: one launch per accepted step, state in HBM, no refill, full
@@ -152,6 +156,9 @@ struct KernelSet {
     int flop_rhs, flop_vjp;
     hipError_t (*fwd)(const FwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*rev)(const RevArgs &, int grid, int block, hipStream_t);
+    // forward / reverse of the long-running instances, one per wave (null where absent)
+    hipError_t (*long_fwd)(const FwdArgs &, int grid, int block, hipStream_t);
+    hipError_t (*long_rev)(const RevArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_fwd)(const PlainFwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_rev)(const PlainRevArgs &, int grid, int block, hipStream_t);
     int (*occ_fwd)(int block);   // resident blocks per CU of the persistent kernels
