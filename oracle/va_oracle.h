@@ -43,11 +43,13 @@
  *                     grid t0 + n dt, FSAL evaluation count, status paths
  *   controller ...... test_oracle_forward::test_controller_invariants (accepted
  *                     err <= 1, shrink in [0.2, 1), growth bounds, dead band,
- *                     landing on t1) and the shrink-exponent mutant
+ *                     landing on t1) and the shrink-exponent mutant; its root
+ *                     x^(-1/q): test_oracle_math against the exact root (40-digit
+ *                     Decimal, <= 1 ulp) and the C library pow (<= 4 ulp)
  *   adjoint ......... test_oracle_adjoint: complex-step derivative of the frozen
  *                     step map, oscillator transfer matrix, central differences,
  *                     the reference's seed semantics, bitwise replay
- *   mutants ......... test_oracle_mutants: five planted mistakes, each caught
+ *   mutants ......... test_oracle_mutants: six planted mistakes, each caught
  * Parity unpinned: the exact accept/reject decisions of Boost.odeint 1.74
  * itself (Boost is not installed, and the reference holds no recorded
  * outputs); the restated controller is pinned only by the invariants above.
