@@ -370,9 +370,9 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
 }
 
 // va_solve with the instances that ran long in the previous solve (the head of
-// order_fwd, counted in counters[5]) on a second stream, one wave each (their
-// forwards, then their reverses), while the persistent kernels run the forward
-// and then the reverse of the rest of the batch on the caller's stream.  The
+// order_fwd, counted in counters[5]) one wave each, forward then reverse, on
+// the caller's stream, while the persistent kernels run the forward and then
+// the reverse of the rest of the batch on a second stream forked from it.  The
 // persistent kernels leave room on the device for the long-instance waves.
 bool split_solve(const va_handle *h)
 {
@@ -415,18 +415,20 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.order = h->order_fwd;
     ra.split = h->counters + 5;
     ra.part = 1;
+    // The long-instance kernel goes on the caller's stream right after the fork
+    // point, ahead of the persistent kernels (which first wait for the fork on the
+    // second stream), so that its waves are placed first.
     VA_HIP(hipEventRecord(h->ev_fork, s));
+    VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX / 4, 256, s));
     VA_HIP(hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    VA_HIP(h->ks.long_fwd(fa, VA_LONG_MAX / 4, 256, h->side));
-    VA_HIP(h->ks.long_rev(ra, VA_LONG_MAX / 4, 256, h->side));
-    VA_HIP(hipEventRecord(h->ev_join, h->side));
     // each persistent kernel leaves the room of VA_LONG_MAX waves of its own size
     const int per_block = h->ks.slots(h->block);
     const int reserve = blocks_for((int64_t)VA_LONG_MAX * h->ks.slots(64), per_block);
     const int gf = blocks_for(std::min<int64_t>(h->lanes_fwd, B), per_block);
     const int gr = blocks_for(std::min<int64_t>(h->lanes_rev, B), per_block);
-    VA_HIP(h->ks.fwd(fa, std::max(1, gf - reserve), h->block, s));
-    VA_HIP(h->ks.rev(ra, std::max(1, gr - reserve), h->block, s));
+    VA_HIP(h->ks.fwd(fa, std::max(1, gf - reserve), h->block, h->side));
+    VA_HIP(h->ks.rev(ra, std::max(1, gr - reserve), h->block, h->side));
+    VA_HIP(hipEventRecord(h->ev_join, h->side));
     VA_HIP(hipStreamWaitEvent(s, h->ev_join, 0));
     return order_by_length(h, 1, s);
 }
@@ -727,7 +729,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
     }
     // the second stream and its events of split solves (created here: va_solve may
     // run under stream capture)
-    if (e == hipSuccess && ks.long_fwd && pb->adaptive && pb->mode == VA_MODE_PERSISTENT) {
+    if (e == hipSuccess && ks.long_fused && pb->adaptive && pb->mode == VA_MODE_PERSISTENT) {
         e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);

@@ -567,27 +567,22 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 
 // ---------------------------------------------------------------------------
 // Instances expected to run long: queue positions [0, *split) of the forward
-// order, one instance per wave (group 0), in two launches on a stream of their
-// own -- all forwards, then all reverses -- while the persistent kernels run the
-// rest of the batch (positions from *split) on the caller's stream.  A wave
-// never waits for another instance, and the long chains start at once instead
-// of wherever the queue would reach them.  (Two launches rather than one fused
-// kernel: each fits beside the persistent kernel of its phase on a CU.)
+// order, one instance per wave (group 0).  Each wave runs its instance's
+// forward and then at once its reverse, so it never waits for another
+// instance; the rest of the batch (positions from *split) runs in the
+// persistent kernels on another stream meanwhile.
 // ---------------------------------------------------------------------------
 template <int NN, class Tab>
-__global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs a)
+__global__ void __launch_bounds__(256) k_glv_rows_long(FwdArgs fa, RevArgs ra)
 {
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= *a.split || w >= a.sp.B) return;
-    glv_rows_fwd_body<NN, Tab, true, true>(a, a.order[w]);
-}
-
-template <int NN, class Tab>
-__global__ void __launch_bounds__(256) k_glv_rows_long_rev(RevArgs a)
-{
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= *a.split || w >= a.sp.B) return;
-    glv_rows_rev_body<NN, Tab, true>(a, a.order[w]);
+    if (w >= *fa.split || w >= fa.sp.B) return;
+    const int64_t b = fa.order[w];
+    glv_rows_fwd_body<NN, Tab, true, true>(fa, b);
+    // the forward's rows, y(T) and status (written by other lanes of the group)
+    // become visible to the reverse's loads
+    __threadfence();
+    glv_rows_rev_body<NN, Tab, true>(ra, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -807,14 +802,10 @@ struct LaunchGlvRows {
         hipLaunchKernelGGL((k_glv_rows_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
-    static hipError_t lfwd(const FwdArgs &a, int grid, int block, hipStream_t s)
+    static hipError_t lng(const FwdArgs &fa, const RevArgs &ra, int grid, int block,
+                          hipStream_t s)
     {
-        hipLaunchKernelGGL((k_glv_rows_long_fwd<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
-        return hipGetLastError();
-    }
-    static hipError_t lrev(const RevArgs &a, int grid, int block, hipStream_t s)
-    {
-        hipLaunchKernelGGL((k_glv_rows_long_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_glv_rows_long<NN, Tab>), dim3(grid), dim3(block), 0, s, fa, ra);
         return hipGetLastError();
     }
     static hipError_t pfwd(const PlainFwdArgs &a, int grid, int block, hipStream_t s)
@@ -856,10 +847,7 @@ struct LaunchGlvRows {
         k->occ_rev = &occ_rev;
         k->slots = &slots;
         k->layout = 1;
-        if constexpr (Ad) {
-            k->long_fwd = &lfwd;
-            k->long_rev = &lrev;
-        }
+        if constexpr (Ad) k->long_fused = &lng;
     }
 };
 
