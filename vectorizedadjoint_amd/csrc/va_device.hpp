@@ -299,9 +299,12 @@ struct FivePow {
 //   accept, err < 0.5:  h * 0.9 max(5^-ORDER, err)^(-1/ORDER)
 //   accept otherwise:   h
 // with ONE root evaluation: its argument and order are selected first, so lanes
-// of a wave that take different decisions share one instruction stream.
+// of a wave that take different decisions share one instruction stream.  When
+// every active lane of the wave takes the same decision (a wave-uniform branch)
+// only that order's root is formed; otherwise both orders' candidates are
+// formed and each lane selects its own.  The lane's value is the same either way.
 template <class Tab>
-VA_HD bool control_step(double h, double err, double &h_next)
+__device__ __forceinline__ bool control_step(double h, double err, double &h_next)
 {
     constexpr double lo = FivePow<Tab::ORDER>::inv();
     // fixed-step tableaus (no error estimate) never reach the controller
@@ -309,7 +312,13 @@ VA_HD bool control_step(double h, double err, double &h_next)
     constexpr int QD = Tab::ERR_ORDER >= 3 ? Tab::ERR_ORDER - 1 : 2;
     const bool accept = !(err > 1.0);
     const double base = accept ? ((lo < err) ? err : lo) : err;
-    const double f = 0.9 * ctrl_root<QI, QD>(base, accept);
+    const uint64_t all = __builtin_amdgcn_ballot_w64(true);
+    const uint64_t acc = __builtin_amdgcn_ballot_w64(accept);
+    double root;
+    if (acc == all) root = ctrl_root<QI, QI>(base, true);
+    else if (acc == 0) root = ctrl_root<QD, QD>(base, false);
+    else root = ctrl_root<QI, QD>(base, accept);
+    const double f = 0.9 * root;
     if (!accept) {
         h_next = h * ((f < 0.2) ? 0.2 : f);
     } else {
