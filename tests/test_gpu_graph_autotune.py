@@ -23,18 +23,21 @@ def _same(a, b):
         assert np.array_equal(a[k], b[k], equal_nan=True), k
 
 
-@pytest.mark.parametrize("name", ["glv10", "vdp"])
-def test_graph_flag_replays_match_eager(name):
+# lanes=120: fewer instances in flight than the batch, so repeated solves reuse the
+# order and run the long instances on the handle's second stream inside the graph
+@pytest.mark.parametrize("name,lanes", [("glv10", 0), ("vdp", 0), ("glv10", 120)])
+def test_graph_flag_replays_match_eager(name, lanes):
     import torch
     import vectorizedadjoint_amd as va
     B = 2048
     w, y0h, ph = _inputs(name, B)
     y0 = torch.from_numpy(y0h).cuda()
     p = torch.from_numpy(ph).cuda()
-    eager = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024))
-    graph = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024, flags=va.FLAG_GRAPH))
+    eager = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024, lanes=lanes))
+    graph = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024, lanes=lanes,
+                                               flags=va.FLAG_GRAPH))
     out_g = None
-    print(f"[gpu] graph replay {name} B={B}", flush=True)
+    print(f"[gpu] graph replay {name} B={B} lanes={lanes}", flush=True)
     for it in range(4):
         # new data in the same buffers between replays, with eager work in between
         _, y2, p2 = _inputs(name, B, seed_shift=it * 7)
@@ -55,14 +58,16 @@ def test_graph_flag_replays_match_eager(name):
     graph.close()
 
 
-def test_torch_graph_capture_of_solve():
+@pytest.mark.parametrize("lanes", [0, 120])
+def test_torch_graph_capture_of_solve(lanes):
     import torch
     import vectorizedadjoint_amd as va
     B = 1024
     w, y0h, ph = _inputs("glv10", B)
     y0 = torch.from_numpy(y0h).cuda()
     p = torch.from_numpy(ph).cuda()
-    s = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024))
+    print(f"[gpu] torch graph capture glv10 B={B} lanes={lanes}", flush=True)
+    s = va.Solver(va.Problem.from_workload(w, batch=B, max_steps=1024, lanes=lanes))
     ref = s.solve(y0, p, w.cost)
     torch.cuda.synchronize()
     ref = _np(ref)
