@@ -310,11 +310,15 @@ va_status order_by_length(va_handle *h, int mode, hipStream_t s)
 
 va_status order_longest_first(va_handle *h, hipStream_t s) { return order_by_length(h, 0, s); }
 
-// The forward order is used when instances are refilled (more instances than
-// are in flight) and the caller did not ask for index order.
+// The forward order is used unless the caller asked for index order: where
+// instances are refilled (more instances than are in flight), so that the long
+// ones start first, and with one lane per instance also when all are resident,
+// because the 64 instances that share a wave then have similar lengths (a wave
+// runs as long as its longest instance).
 bool reuse_order(const va_handle *h, int64_t lanes)
 {
-    return !(h->pb.flags & VA_FLAG_COLD_ORDER) && h->pb.batch > lanes;
+    if (h->pb.flags & VA_FLAG_COLD_ORDER) return false;
+    return h->pb.batch > lanes || (h->ks.layout == 0 && h->pb.batch > 64);
 }
 
 // ---------------- persistent form ----------------
