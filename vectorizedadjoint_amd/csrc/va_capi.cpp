@@ -88,13 +88,14 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
 
 // Longest-first orders.  The reverse of one instance is a chain of n_acc
 // dependent steps, its forward a chain of n_acc + n_rej attempts; when the long
-// chains start last they set the end of the launch.  Instances are bucketed by
-// the chain length (buckets of 16 below 512, of 256 above, the last
+// chains start last they set the end of the launch, and with one lane per
+// instance a wave runs as long as its longest instance.  Instances are bucketed
+// by the chain length (buckets of 8 below 1024, of 64 above, the last
 // open-ended) and taken bucket by bucket from the longest; within a bucket the
 // order is arbitrary.  Reverse order (mode 0): accepted steps, failed instances
 // last (they only write NaN).  Forward order (mode 1): attempts, every status.
 // Each instance's result does not depend on the order.
-constexpr int VA_ORDER_BUCKETS = 64;
+constexpr int VA_ORDER_BUCKETS = 256;  // = the block size of the order kernels
 
 __device__ __forceinline__ int order_bucket(const int32_t *n_acc, const int32_t *n_rej,
                                             const int32_t *status, int64_t b, int mode)
@@ -102,7 +103,7 @@ __device__ __forceinline__ int order_bucket(const int32_t *n_acc, const int32_t 
     int32_t len = n_acc[b];
     if (mode == 0 && status[b] != 0) return 0;
     if (mode == 1) len += n_rej[b];
-    const int k = len < 512 ? (len >> 4) : 32 + ((len - 512) >> 8);
+    const int k = len < 1024 ? (len >> 3) : 128 + ((len - 1024) >> 6);
     return k < VA_ORDER_BUCKETS - 1 ? k : VA_ORDER_BUCKETS - 1;
 }
 
@@ -123,7 +124,7 @@ __global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const 
 // cursor[k] = number of instances in buckets above k (longest bucket first).
 // With n_long: the instances in buckets >= VA_LONG_BUCKET, at most VA_LONG_MAX
 // (the leading positions of the order that the long-instance kernel takes).
-constexpr int VA_LONG_BUCKET = 16;  // 256 attempts and more
+constexpr int VA_LONG_BUCKET = 32;  // 256 attempts and more
 constexpr int VA_LONG_MAX = 256;    // one wave each
 
 __global__ void k_order_offsets(const int32_t *hist, int32_t *cursor, int32_t *n_long)
