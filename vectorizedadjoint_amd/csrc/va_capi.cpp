@@ -127,16 +127,23 @@ __global__ void __launch_bounds__(256) k_order_hist(const int32_t *n_acc, const 
 constexpr int VA_LONG_BUCKET = 32;  // 256 attempts and more
 constexpr int VA_LONG_MAX = 256;    // one wave each
 
-__global__ void k_order_offsets(const int32_t *hist, int32_t *cursor, int32_t *n_long)
+__global__ void __launch_bounds__(VA_ORDER_BUCKETS)
+k_order_offsets(const int32_t *hist, int32_t *cursor, int32_t *n_long)
 {
-    if (threadIdx.x == 0) {
-        int32_t acc = 0;
-        for (int k = VA_ORDER_BUCKETS - 1; k >= 0; --k) {
-            if (k == VA_LONG_BUCKET - 1 && n_long) *n_long = acc < VA_LONG_MAX ? acc : VA_LONG_MAX;
-            cursor[k] = acc;
-            acc += hist[k];
-        }
+    // one thread per bucket: suffix sums by doubling steps
+    __shared__ int32_t sfx[VA_ORDER_BUCKETS];
+    const int k = threadIdx.x;
+    const int32_t own = hist[k];
+    sfx[k] = own;
+    __syncthreads();
+    for (int d = 1; d < VA_ORDER_BUCKETS; d *= 2) {
+        const int32_t v = k + d < VA_ORDER_BUCKETS ? sfx[k + d] : 0;
+        __syncthreads();
+        sfx[k] += v;
+        __syncthreads();
     }
+    cursor[k] = sfx[k] - own;
+    if (k == VA_LONG_BUCKET && n_long) *n_long = sfx[k] < VA_LONG_MAX ? sfx[k] : VA_LONG_MAX;
 }
 
 __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *n_acc,
@@ -301,7 +308,7 @@ va_status order_by_length(va_handle *h, int mode, hipStream_t s)
     int32_t *out = mode == 0 ? h->order : h->order_fwd;
     hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej, h->status,
                        B, mode, h->order_tmp);
-    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(64), 0, s, h->order_tmp,
+    hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(VA_ORDER_BUCKETS), 0, s, h->order_tmp,
                        h->order_tmp + VA_ORDER_BUCKETS, mode == 1 ? h->counters + 5 : nullptr);
     hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej,
                        h->status, B, mode, h->order_tmp + VA_ORDER_BUCKETS, out);
