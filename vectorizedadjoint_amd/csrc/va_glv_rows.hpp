@@ -76,7 +76,9 @@ __device__ __forceinline__ double group_max(const double (&v)[NN])
 
 // Long-running instances set the job's critical path: a wave holding one gets
 // SIMD issue priority so it runs near its lone-wave latency while other waves
-// fill the remaining issue slots.
+// fill the remaining issue slots.  (The one-instance-per-wave long kernel runs
+// at the highest priority throughout; persistent kernels running beside it,
+// i.e. with a queue split, leave their priority at 0.)
 __device__ __forceinline__ void prioritise_long(bool long_running)
 {
     if (__builtin_amdgcn_ballot_w64(long_running) != 0)
@@ -203,7 +205,8 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
     const QueueRange q = queue_range(B, a.split, a.part);
 
     for (;;) {
-        prioritise_long(OneShot || (!need && count + rej > VA_LONG_ATTEMPTS));
+        if constexpr (OneShot) __builtin_amdgcn_s_setprio(3);
+        else if (!a.split) prioritise_long(!need && count + rej > VA_LONG_ATTEMPTS);
         if (need) {
             if constexpr (OneShot) {
                 if (taken) break;
@@ -391,7 +394,8 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
     const QueueRange q = queue_range(B, a.split, a.part);
 
     for (;;) {
-        prioritise_long(OneShot || (!need && nr > VA_LONG_ATTEMPTS));
+        if constexpr (OneShot) __builtin_amdgcn_s_setprio(3);
+        else if (!a.split) prioritise_long(!need && nr > VA_LONG_ATTEMPTS);
         if (need) {
             if constexpr (OneShot) {
                 if (taken) break;
