@@ -122,6 +122,26 @@ __device__ __forceinline__ void exchange2(double *slab_a, double *slab_b, int la
     __builtin_amdgcn_wave_barrier();
 }
 
+// Group 0 of a wave (lanes 0..N-1) running alone: each lane's value reaches the
+// group through scalar registers (v_readlane) instead of an LDS round trip.
+template <int NN>
+__device__ __forceinline__ void bcast_group0(double x, double (&out)[NN])
+{
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+#pragma unroll
+    for (int j = 0; j < NN; ++j)
+        out[j] = __hiloint2double(__builtin_amdgcn_readlane(hi, j),
+                                  __builtin_amdgcn_readlane(lo, j));
+}
+
+template <bool OneShot, int NN>
+__device__ __forceinline__ void xchg(double *slab, int lane, int gbase, double x,
+                                     double (&out)[NN])
+{
+    if constexpr (OneShot) bcast_group0<NN>(x, out);
+    else exchange<NN>(slab, lane, gbase, x, out);
+}
+
 // g_i = r_i + sum_j A_ij x_j: two interleaved partial sums (even j from r_i,
 // odd j from 0) added at the end -- half the dependent-FMA depth, same order as
 // the one-lane kernels and the oracle.
@@ -249,7 +269,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 row[2 + i] = y;
                 if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
                 if ((!Tab::FSAL && !kSpec) || !have_k0) {
-                    exchange<NN>(slab, lane, gbase, y, v);
+                    xchg<OneShot, NN>(slab, lane, gbase, y, v);
                     K[0] = y * growth_row<NN>(Arow, r, v);
                     have_k0 = true;
                 }
@@ -265,7 +285,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
 #pragma unroll
                 for (int j = 0; j < Tab::SF; ++j)
                     if (j < m && Tab::A[m][j] != 0.0) X = fma(Tab::A[m][j] * h, K[j], X);
-                exchange<NN>(slab, lane, gbase, X, v);
+                xchg<OneShot, NN>(slab, lane, gbase, X, v);
                 K[m] = X * growth_row<NN>(Arow, r, v);
             }
             ynew = y;
@@ -280,10 +300,15 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
                 double k0n = 0.0, ve[NN];
                 if constexpr (kSpec) {
-                    exchange2<NN>(slab, slabb, lane, gbase, ynew, v, fabs(xerr) / den, ve);
+                    if constexpr (OneShot) {
+                        bcast_group0<NN>(ynew, v);
+                        bcast_group0<NN>(fabs(xerr) / den, ve);
+                    } else {
+                        exchange2<NN>(slab, slabb, lane, gbase, ynew, v, fabs(xerr) / den, ve);
+                    }
                     k0n = ynew * growth_row<NN>(Arow, r, v);
                 } else {
-                    exchange<NN>(slabb, lane, gbase, fabs(xerr) / den, ve);
+                    xchg<OneShot, NN>(slabb, lane, gbase, fabs(xerr) / den, ve);
                 }
                 const double err = group_max<NN>(ve);
                 double dt_next;
@@ -321,7 +346,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                     row[1] = 0.0;
                 }
                 row[2 + i] = y;
-                exchange<NN>(slab, lane, gbase, fabs(y), v);
+                xchg<OneShot, NN>(slab, lane, gbase, fabs(y), v);
                 bool fin = true;
 #pragma unroll
                 for (int j = 0; j < NN; ++j) fin = fin && (v[j] <= 1.7976931348623157e308);
@@ -431,7 +456,7 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
             if (a.cost) {
                 const double yT = a.yT[(int64_t)i * B + b];
                 // terminal cost: J = fold over i in order, lambda_i from y_i(T)
-                exchange<NN>(uslab, lane, gbase, yT, v);
+                xchg<OneShot, NN>(uslab, lane, gbase, yT, v);
                 double J = 0.0;
 #pragma unroll
                 for (int j = 0; j < NN; ++j) {
@@ -477,7 +502,12 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                     for (int j = 0; j < S; ++j)
                         if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * hr, Kn[j], x);
                     double *xs = reinterpret_cast<double *>(xsl2[wv][pr][m]);
-                    exchange<NN>(xs, lane, gbase, x, v);
+                    if constexpr (OneShot) {
+                        xs[lane] = x;
+                        bcast_group0<NN>(x, v);
+                    } else {
+                        exchange<NN>(xs, lane, gbase, x, v);
+                    }
                     const double g = growth_row<NN>(Arow, r, v);
                     gr[m * 64 + lane] = g;
                     Kn[m] = x * g;
@@ -502,11 +532,16 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                     double *xs = reinterpret_cast<double *>(xsl2[wv][pr][ii]);
                     double vu[NN];
                     xs[lane] = x;
-                    uslab[lane] = u;
-                    __builtin_amdgcn_wave_barrier();
-                    gather_vec<NN>(xs, gbase, v);
-                    gather_vec<NN>(uslab, gbase, vu);
-                    __builtin_amdgcn_wave_barrier();
+                    if constexpr (OneShot) {
+                        bcast_group0<NN>(x, v);
+                        bcast_group0<NN>(u, vu);
+                    } else {
+                        uslab[lane] = u;
+                        __builtin_amdgcn_wave_barrier();
+                        gather_vec<NN>(xs, gbase, v);
+                        gather_vec<NN>(uslab, gbase, vu);
+                        __builtin_amdgcn_wave_barrier();
+                    }
                     const double g = growth_row<NN>(Arow, r, v);
                     gr[ii * 64 + lane] = g;
                     Kn[ii] = x * g;
