@@ -34,3 +34,13 @@ def test_problem_struct_layout_matches_header():
     assert va.VaProblem.lanes.offset == 104
     assert va.VaProblem.trajectory_budget_bytes.offset == 112
     assert va.VaProblem.n_obs.offset == 120 and va.VaProblem.checkpoint_slots.offset == 100
+
+
+def test_header_constants_match_the_python_mirror():
+    # every #define VA_<NAME> <int> of the header has the same value in constants.py
+    from vectorizedadjoint_amd import constants as K
+    src = open(os.path.join(ROOT, "include", "vectorizedadjoint_amd.h")).read()
+    pairs = re.findall(r"#define VA_(FLAG_[A-Z_]+)\s+(\d+)", src)
+    assert {name for name, _ in pairs} >= {"FLAG_GRAPH", "FLAG_COLD_ORDER"}
+    for name, value in pairs:
+        assert getattr(K, name) == int(value), name
