@@ -701,6 +701,24 @@ typedef struct {
     double xi[ORC_MAX_N];
 } orc_rwork;
 
+/* pbar += the stage's parameter VJP.  For Lotka-Volterra (per-instance A) the
+ * entries A_ij get u_i x_j with one fused multiply-add each (wjp holds u_i in
+ * its first n entries), the accumulation the HIP kernels use; every other
+ * parameter adds its rounded VJP term. */
+static void accumulate_param_bar(int system, int n, int np, const double *x, const double *wjp,
+                                 double *pbar)
+{
+    if (system == ORC_SYS_LOTKA_VOLTERRA) {
+        for (int i = 0; i < n; ++i) {
+            pbar[i] += wjp[i];
+            double *Ab = pbar + (size_t)n * (i + 1);
+            for (int j = 0; j < n; ++j) Ab[j] = fma(wjp[i], x[j], Ab[j]);
+        }
+        return;
+    }
+    for (int k = 0; k < np; ++k) pbar[k] += wjp[k];
+}
+
 static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const double *p,
                            const orc_trajectory *tr, int64_t nidx, double *lambda,
                            double *pbar, double *wjp, orc_rwork *w)
@@ -739,7 +757,7 @@ static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const d
             for (int i = 0; i < n; ++i) w->Kb[j][i] = fma(ah, w->xi[i], w->Kb[j][i]);
 #endif
         }
-        for (int k = 0; k < np; ++k) pbar[k] += wjp[k];
+        accumulate_param_bar(pb->system, n, np, w->X[m], wjp, pbar);
     }
     for (int i = 0; i < n; ++i) lambda[i] = w->ybar[i];
 }

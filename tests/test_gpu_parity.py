@@ -4,9 +4,9 @@ seeded inputs.
 Bar (per instance): identical accepted/rejected step counts; y(T) equal to the
 oracle (the forward pass follows the oracle's IEEE operation sequence, so it
 is compared bit for bit); J, dJ/dy0, dJ/dp within a norm-wise relative error
-of 1e-10 (the stated floating-point tolerance).  The persistent one-lane
-kernels also follow the oracle's reverse sequence, so their gradients are
-checked at 1e-13 as well.
+of 1e-10 (the stated floating-point tolerance).  The persistent kernels also
+follow the oracle's reverse sequence, so their gradients are checked at
+GRAD_TIGHT = 1e-13 as well.
 """
 import numpy as np
 import pytest
@@ -16,6 +16,7 @@ from helpers import rel_err, run_oracle
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
+GRAD_TIGHT = 1e-13
 
 
 def _torch():
@@ -72,7 +73,7 @@ def test_persistent_matches_oracle(name, B):
     inp = I.make_inputs(w, B)
     o = run_oracle(w, inp, max_steps=4096)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096)
-    _compare(g, o, grad_tol=1e-13)
+    _compare(g, o, grad_tol=GRAD_TIGHT)
 
 
 @pytest.mark.parametrize("name,B", CASES)
@@ -94,7 +95,7 @@ def test_refill_with_few_lanes_matches_oracle(ipw):
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, lanes=120, block_size=64,
                    instances_per_wave=ipw)
-    _compare(g, o, grad_tol=1e-13)
+    _compare(g, o, grad_tol=GRAD_TIGHT)
 
 
 @pytest.mark.parametrize("ipw", [64, 6])
@@ -109,7 +110,7 @@ def test_glv_layouts_and_tableaus_match_oracle(ipw, tableau):
     inp = I.make_inputs(w, 600)
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=ipw)
-    _compare(g, o, grad_tol=1e-13)
+    _compare(g, o, grad_tol=GRAD_TIGHT)
 
 
 def test_glv_fixed_step_rk4_rows_matches_oracle():
@@ -120,4 +121,4 @@ def test_glv_fixed_step_rk4_rows_matches_oracle():
     inp = I.make_inputs(w, 300)
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=6)
-    _compare(g, o, grad_tol=1e-13)
+    _compare(g, o, grad_tol=GRAD_TIGHT)

@@ -467,7 +467,8 @@ struct LotkaVolterra {
         for (int i = 0; i < NN; ++i) {
             pbar[i] = pbar[i] + u[i];
 #pragma unroll
-            for (int j = 0; j < NN; ++j) pbar[NN * (i + 1) + j] = pbar[NN * (i + 1) + j] + u[i] * x[j];
+            for (int j = 0; j < NN; ++j)
+                pbar[NN * (i + 1) + j] = fma(u[i], x[j], pbar[NN * (i + 1) + j]);
         }
     }
 };

@@ -382,7 +382,8 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 // lanes' g go to LDS (two buffers per wave, alternating steps), where the
 // backward of the next iteration reads them.  The first iteration of an
 // instance (or of a further seed) only recomputes.  Every value is formed by
-// the same operations in the same order as in the unpipelined sweep.
+// the same operations in the same order as in the oracle (dJ/dA: one fused
+// multiply-add per product u_k X_i).
 // ---------------------------------------------------------------------------
 template <int NN, class Tab, bool OneShot>
 __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
@@ -548,9 +549,9 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                     const double xi = adjoint_col<NN>(Acol, wg, vu);
                     rbar = rbar + u;
                     // column i of dJ/dA: dA_ki += u_k X_i, from the gathered u-vector and
-                    // the lane's own X (no X-vector gather)
+                    // the lane's own X (no X-vector gather), one fused multiply-add each
 #pragma unroll
-                    for (int k = 0; k < NN; ++k) Abar[k] = Abar[k] + vu[k] * xown;
+                    for (int k = 0; k < NN; ++k) Abar[k] = fma(vu[k], xown, Abar[k]);
                     ybar = ybar + xi;
 #pragma unroll
                     for (int j = 0; j < S; ++j)
@@ -817,7 +818,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
         rbar = rbar + u;
         gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
 #pragma unroll
-        for (int j = 0; j < NN; ++j) Abar[j] = Abar[j] + u * v[j];
+        for (int j = 0; j < NN; ++j) Abar[j] = fma(u, v[j], Abar[j]);
         ybar = ybar + xi;
 #pragma unroll
         for (int j = 0; j < S; ++j)
