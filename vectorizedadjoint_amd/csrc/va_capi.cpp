@@ -200,6 +200,7 @@ struct va_handle {
     // shared-parameter solves
     int shared_grid = 0;
     double *abar_part = nullptr;
+    double *shared_gscr = nullptr;  // per-workgroup growth products of the shared-A reverse
     // VA_FLAG_GRAPH: va_solve recorded once per argument set, then replayed
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -289,6 +290,7 @@ void release(va_handle *h)
     hipFree(h->obs_stride);
     hipFree(h->obs_row);
     hipFree(h->abar_part);
+    hipFree(h->shared_gscr);
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->cap_stream) hipStreamDestroy(h->cap_stream);
     if (h->side) hipStreamDestroy(h->side);
@@ -721,6 +723,9 @@ va_status va_create(const va_problem *pb, va_handle **out)
     }
     if (ks.layout == 2)
         alloc((void **)&h->abar_part, sizeof(double) * (size_t)h->shared_grid * ks.n * ks.n);
+        // per workgroup: waves x s stages x 64 lanes x 4 growth values
+        alloc((void **)&h->shared_gscr, sizeof(double) * (size_t)h->shared_grid *
+                                            ks.shared_waves * ks.s_rev * 64 * 4);
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {
@@ -1081,6 +1086,7 @@ va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const
     r.dJdy0 = dJdy0;
     r.dJdr = dJdp;
     r.abar_part = h->abar_part;
+    r.gscr = h->shared_gscr;
     VA_HIP(h->ks.shared_rev(r, h->shared_grid, s));
     VA_HIP(va::launch_glvs_reduce(h->abar_part, h->shared_grid, dJdp_shared, s));
     return copy_out(h, n_accepted, n_rejected, lane_status, s);

@@ -314,6 +314,8 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
 #pragma unroll
     for (int jb = 0; jb < GS_N / 16; ++jb) abar[jb] = d4{0.0, 0.0, 0.0, 0.0};
     const int64_t ntiles = (B + GS_T - 1) / GS_T;
+    // growth products of the S stages of the step being reversed, [S][64 lanes] per wave
+    d4 *gw = reinterpret_cast<d4 *>(a.gscr) + ((size_t)blockIdx.x * (GS_THREADS / 64) + w) * S * 64;
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         int64_t b[4];
@@ -353,8 +355,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                     yn[jj] = row[2 + comp];
                 }
             }
-            // recompute the stages from y_n (backpropagation.hpp:24-64); only K_m is
-            // kept: X_m is rebuilt from it and g_m recomputed where the adjoint needs them
+            // recompute the stages from y_n (backpropagation.hpp:24-64); K_m is kept in
+            // registers (X_m is rebuilt from it), the growth products A X_m go to this
+            // wave's scratch rows, where the adjoint reads them back
             for (int m = 0; m < S; ++m) {
                 double X[4];
 #pragma unroll
@@ -365,6 +368,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                 __syncthreads();
                 const d4 G = gemm_xat(Xs, As, w, r, c);
                 __syncthreads();
+                gw[m * 64 + lane] = G;
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) K[m][jj] = X[jj] * (rr[jj] + G[jj]);
             }
@@ -376,6 +380,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                 for (int m = 0; m < S; ++m) Kb[m][jj] = (Tab::B[m] * h[jj]) * lam[jj];
             }
             for (int m = S - 1; m >= 0; --m) {
+                const d4 G = gw[m * 64 + lane];  // this lane's own growth products of stage m
                 double u[4];
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
@@ -385,7 +390,6 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                     Xs[sw(r + 4 * jj, comp)] = X;
                 }
                 __syncthreads();
-                const d4 G = gemm_xat(Xs, As, w, r, c);  // g_m again, bit-identical
                 const d4 xu = gemm_ua(Us, As, w, r, c);
                 gemm_utx(Us, Xs, w, r, c, abar);
                 __syncthreads();
@@ -462,16 +466,19 @@ bool kernels_lotka_volterra_shared(int n, int tableau, int adaptive, KernelSet *
     k.flop_vjp = 4 * GS_N * GS_N + 3 * GS_N;
     k.layout = 2;
     k.shared_tile = GS_T;
+    k.shared_waves = GS_WAVES;
     switch (tableau) {
     case TAB_CK54:
         k.shared_fwd = &LaunchShared<CashKarp54>::fwd;
         k.shared_rev = &LaunchShared<CashKarp54>::rev;
         k.s_fwd = CashKarp54::SF;
+        k.s_rev = CashKarp54::S;
         break;
     case TAB_DOPRI5:
         k.shared_fwd = &LaunchShared<Dopri5>::fwd;
         k.shared_rev = &LaunchShared<Dopri5>::rev;
         k.s_fwd = Dopri5::SF;
+        k.s_rev = Dopri5::S;
         break;
     default: return false;
     }

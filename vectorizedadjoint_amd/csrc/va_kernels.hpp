@@ -149,6 +149,7 @@ struct SharedRevArgs {
     double *dJdy0;         // [N][B]
     double *dJdr;          // [N][B]
     double *abar_part;     // [grid][N][N] per-workgroup partial dJ/dA
+    double *gscr;          // [grid][waves][S][64][4] growth products of the step being reversed
 };
 
 struct KernelSet {
@@ -173,6 +174,7 @@ struct KernelSet {
     hipError_t (*shared_fwd)(const SharedFwdArgs &, int grid, hipStream_t);
     hipError_t (*shared_rev)(const SharedRevArgs &, int grid, hipStream_t);
     int shared_tile;
+    int shared_waves;            // waves per workgroup of the shared-parameter kernels
 };
 
 hipError_t launch_glvs_reduce(const double *part, int nparts, double *out, hipStream_t s);
