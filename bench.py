@@ -206,11 +206,12 @@ def bench_shared(a, torch, va, I):
     n_ok = int((st == 0).sum())
     N = w.n
     # MFMA work actually issued: 6 growth products per attempt (5 stages + f(y_{n+1})),
-    # 6 recompute + 6 x 3 adjoint products per reverse step; 2 N^2 FLOP per instance each
+    # 6 recompute + 6 x 2 adjoint products (U A, U^T X) per reverse step; 2 N^2 FLOP
+    # per instance each
     gemm = 2.0 * N * N
     attempts = float(np.sum(n_acc + n_rej))
     steps = float(np.sum(n_acc[st == 0]))
-    fl_issued = gemm * (6.0 * attempts + 24.0 * steps)
+    fl_issued = gemm * (6.0 * attempts + 18.0 * steps)
     cpu = None
     if not a.no_cpu:
         log("[bench] CPU baseline (oracle, shared A)")
