@@ -205,7 +205,11 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 // ---------------------------------------------------------------------------
 
 // Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg) from the checkpoint c, replaying
-// the forward's controlled loop with the same observation stops.
+// the forward's controlled loop with the same observation stops.  One trip of
+// the loop is one attempt, whether the lane starts a step or retries one, so
+// the lanes of a wave do not wait for each other's rejected attempts step by
+// step (with 64 lanes some lane rejects in most steps); the values formed are
+// those of the forward.
 template <class Sys, class Tab>
 __device__ void recompute_segment(const typename Sys::Params &q, const SolveParams &sp,
                                   const double *obs_t, int K, const double *c, int32_t len,
@@ -214,53 +218,66 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
     constexpr int N = Sys::N, FS = N + 1;
     using R = RK<Sys, Tab>;
     double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
-    double t = c[0], dt = c[1];
+    double t = c[0], dt = c[1], dt_keep = 0.0;
     int kobs = (int)c[2];
 #pragma unroll
     for (int i = 0; i < N; ++i) y[i] = c[3 + i];
     // f(y) here equals the FSAL value the forward carried into this step: the
     // last stage input of DOPRI5 is y_{n+1} computed with the same operations.
     Sys::rhs(q, y, Kst[0]);
-    for (int32_t k = 0; k < len; ++k) {
-        double target;
-        bool is_obs;
-        for (;;) {
-            is_obs = kobs < K;
-            target = is_obs ? obs_t[kobs] : sp.t1;
-            if (!is_obs || less_with_sign(t, target, dt)) break;
-            ++kobs;
-        }
-        double *r = seg + (size_t)k * FS;
-#pragma unroll
-        for (int i = 0; i < N; ++i) r[1 + i] = y[i];
-        bool clipped = false;
-        const double dt_keep = dt;
-        if (less_with_sign(target, t + dt, dt)) {
-            dt = target - t;
-            clipped = is_obs;
-        }
-        if (!Tab::FSAL && k > 0) Sys::rhs(q, y, Kst[0]);
-        for (int32_t tries = 0; tries <= sp.max_attempts; ++tries) {
-            const double h = dt;
-            R::attempt(q, y, Kst, h, ynew, xerr);
-            const double err = R::error_norm(sp.atol, sp.rtol, h, y, Kst[0], xerr);
-            double dt_next;
-            const bool ok = control_step<Tab>(dt, err, dt_next);
-            if (ok) {
-                r[0] = dt;
-                t = t + dt;
-#pragma unroll
-                for (int i = 0; i < N; ++i) y[i] = ynew[i];
-                if constexpr (Tab::FSAL) {
-#pragma unroll
-                    for (int i = 0; i < N; ++i) Kst[0][i] = Kst[Tab::SF - 1][i];
-                }
-                if (clipped && fabs(dt_keep) > fabs(dt_next)) dt_next = dt_keep;
-                dt = dt_next;
-                break;
+    int32_t k = 0, tries = 0;
+    bool new_step = true, clipped = false;
+    double *r = seg;
+    while (k < len) {
+        if (new_step) {
+            double target;
+            bool is_obs;
+            for (;;) {
+                is_obs = kobs < K;
+                target = is_obs ? obs_t[kobs] : sp.t1;
+                if (!is_obs || less_with_sign(t, target, dt)) break;
+                ++kobs;
             }
+            r = seg + (size_t)k * FS;
+#pragma unroll
+            for (int i = 0; i < N; ++i) r[1 + i] = y[i];
+            clipped = false;
+            dt_keep = dt;
+            if (less_with_sign(target, t + dt, dt)) {
+                dt = target - t;
+                clipped = is_obs;
+            }
+            if (!Tab::FSAL && k > 0) Sys::rhs(q, y, Kst[0]);
+            tries = 0;
+            new_step = false;
+        }
+        const double h = dt;
+        R::attempt(q, y, Kst, h, ynew, xerr);
+        const double err = R::error_norm(sp.atol, sp.rtol, h, y, Kst[0], xerr);
+        double dt_next;
+        const bool ok = control_step<Tab>(dt, err, dt_next);
+        if (ok) {
+            r[0] = dt;
+            t = t + dt;
+#pragma unroll
+            for (int i = 0; i < N; ++i) y[i] = ynew[i];
+            if constexpr (Tab::FSAL) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) Kst[0][i] = Kst[Tab::SF - 1][i];
+            }
+            if (clipped && fabs(dt_keep) > fabs(dt_next)) dt_next = dt_keep;
+            dt = dt_next;
+            ++k;
+            new_step = true;
+        } else {
             clipped = false;
             dt = dt_next;
+            // (an instance whose forward ended OK never exhausts its tries here;
+            // the bound only keeps the loop finite)
+            if (++tries > sp.max_attempts) {
+                ++k;
+                new_step = true;
+            }
         }
     }
 }
