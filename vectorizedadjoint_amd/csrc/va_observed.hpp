@@ -205,7 +205,9 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 // ---------------------------------------------------------------------------
 
 // Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg) from the checkpoint c, replaying
-// the forward's controlled loop with the same observation stops.  One trip of
+// the forward's controlled loop with the same observation stops.  Field f of
+// row k is seg[(k (N+1) + f) ss]: the lanes' buffers are interleaved (ss = the
+// grid's thread count), so a wave's stores and loads of one field coalesce.  One trip of
 // the loop is one attempt, whether the lane starts a step or retries one, so
 // the lanes of a wave do not wait for each other's rejected attempts step by
 // step (with 64 lanes some lane rejects in most steps); the values formed are
@@ -213,7 +215,7 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 template <class Sys, class Tab>
 __device__ void recompute_segment(const typename Sys::Params &q, const SolveParams &sp,
                                   const double *obs_t, int K, const double *c, int32_t len,
-                                  double *seg)
+                                  double *seg, int64_t ss)
 {
     constexpr int N = Sys::N, FS = N + 1;
     using R = RK<Sys, Tab>;
@@ -227,7 +229,7 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
     Sys::rhs(q, y, Kst[0]);
     int32_t k = 0, tries = 0;
     bool new_step = true, clipped = false;
-    double *r = seg;
+    double *r = seg;  // field 0 of row k
     while (k < len) {
         if (new_step) {
             double target;
@@ -238,9 +240,9 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
                 if (!is_obs || less_with_sign(t, target, dt)) break;
                 ++kobs;
             }
-            r = seg + (size_t)k * FS;
+            r = seg + (size_t)k * FS * ss;
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[1 + i] = y[i];
+            for (int i = 0; i < N; ++i) r[(size_t)(1 + i) * ss] = y[i];
             clipped = false;
             dt_keep = dt;
             if (less_with_sign(target, t + dt, dt)) {
@@ -293,7 +295,9 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
     const double qnan = __builtin_nan("");
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double *seg = nullptr;
-    if constexpr (Ckpt) seg = a.st.seg + (size_t)gtid * (size_t)a.st.stride_max * FS;
+    // lanes' segment buffers interleaved: field f of row k at seg[(k FS + f) ss]
+    const int64_t ss = (int64_t)gridDim.x * blockDim.x;
+    if constexpr (Ckpt) seg = a.st.seg + gtid;
 
     for (;;) {
         int64_t b = atomicAdd(a.counter, 1);
@@ -333,11 +337,15 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
             for (int32_t j = nseg - 1; j >= 0; --j) {
                 const int32_t n0 = j * stride;
                 const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
-                recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, ck + (size_t)j * FC, len, seg);
+                recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, ck + (size_t)j * FC, len, seg,
+                                            ss);
                 for (int32_t k = len - 1; k >= 0; --k) {
-                    const double *r = seg + (size_t)k * FS;
-                    back_prop_step<Sys, Tab>(q, r[0], r + 1, lam, pbar);
-                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, n0 + k, r + 1, lam);
+                    const double *r = seg + (size_t)k * FS * ss;
+                    double yk[N];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) yk[i] = r[(size_t)(1 + i) * ss];
+                    back_prop_step<Sys, Tab>(q, r[0], yk, lam, pbar);
+                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, n0 + k, yk, lam);
                 }
             }
         }
