@@ -57,10 +57,16 @@ __global__ void k_copy_f64(double *dst, const double *src, int64_t n)
     if (i < n) dst[i] = src[i];
 }
 
-__global__ void k_copy_i32(int32_t *dst, const int32_t *src, int64_t n)
+// The per-instance counts and status to the caller's arrays (any may be null).
+__global__ void k_copy_counts(int32_t *n_acc, int32_t *n_rej, int32_t *status,
+                              const int32_t *s_acc, const int32_t *s_rej, const int32_t *s_st,
+                              int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    if (i >= n) return;
+    if (n_acc) n_acc[i] = s_acc[i];
+    if (n_rej) n_rej[i] = s_rej[i];
+    if (status) status[i] = s_st[i];
 }
 
 __global__ void k_identity_seeds(double *lambda, int n, int64_t B)
@@ -533,11 +539,9 @@ va_status copy_out(va_handle *h, int32_t *n_acc, int32_t *n_rej, int32_t *lane_s
                    hipStream_t s)
 {
     const int64_t B = h->pb.batch;
-    const int g = blocks_for(B, 256);
-    if (n_acc) hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, n_acc, h->n_acc, B);
-    if (n_rej) hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, n_rej, h->n_rej, B);
-    if (lane_status)
-        hipLaunchKernelGGL(k_copy_i32, dim3(g), dim3(256), 0, s, lane_status, h->status, B);
+    if (!n_acc && !n_rej && !lane_status) return VA_OK;
+    hipLaunchKernelGGL(k_copy_counts, dim3(blocks_for(B, 256)), dim3(256), 0, s, n_acc, n_rej,
+                       lane_status, h->n_acc, h->n_rej, h->status, B);
     VA_HIP(hipGetLastError());
     return VA_OK;
 }
