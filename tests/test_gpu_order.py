@@ -62,3 +62,53 @@ def test_reused_order_observed(kw):
     assert np.array_equal(outs[0]["n_acc"], o["n_acc"])
     assert np.array_equal(outs[0]["yT"], o["yT"]) and np.array_equal(outs[0]["J"], o["J"])
     _same(outs)
+
+
+def test_separate_calls_split_schedule():
+    """va_forward and va_adjoint on a handle whose batch exceeds the instances in
+    flight: from the second forward on, the instances that ran long run one per
+    wave beside the persistent kernels (the split schedule), in the forward alone
+    and in the reverse alone.  Every output stays bit-identical to va_solve's and
+    to index order; the sensitivity matrix (10 seeds per instance) likewise."""
+    torch = _torch()
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import inputs as I
+    w = I.WORKLOADS["glv10"]
+    B, lanes = 3000, 120
+    inp = I.make_inputs(w, B)
+    o = run_oracle(w, inp, max_steps=1024)
+    # the batch holds instances that ran long (>= 256 attempts), so the split is exercised
+    assert int(((o["n_acc"] + o["n_rej"]) >= 256).sum()) > 0
+    y0, p = _dev(inp["y0"], torch), _dev(inp["p"], torch)
+    N, P = y0.shape[0], p.shape[0]
+    res = {}
+    for flags in (0, va.FLAG_COLD_ORDER):
+        pb = va.Problem.from_workload(w, batch=B, max_steps=1024, lanes=lanes, flags=flags)
+        print(f"[gpu] separate calls glv10 B={B} lanes={lanes} flags={flags}", flush=True)
+        outs = []
+        with va.Solver(pb) as s:
+            ref = {k: v.cpu().numpy() for k, v in s.solve(y0, p, w.cost).items()}
+            for _ in range(3):
+                y = y0.clone()
+                n_acc = torch.empty(B, dtype=torch.int32, device="cuda")
+                n_rej = torch.empty_like(n_acc)
+                st = torch.empty_like(n_acc)
+                s.forward(y, p, n_accepted=n_acc, n_rejected=n_rej, lane_status=st)
+                lam = torch.ones((1, N, B), dtype=torch.float64, device="cuda")
+                mu = torch.zeros((1, P, B), dtype=torch.float64, device="cuda")
+                s.adjoint(p, lam, mu)
+                S = s.sensitivity_matrix(p)
+                torch.cuda.synchronize()
+                outs.append({"yT": y.cpu().numpy(), "dJdy0": lam[0].cpu().numpy(),
+                             "dJdp": mu[0].cpu().numpy(), "n_acc": n_acc.cpu().numpy(),
+                             "n_rej": n_rej.cpu().numpy(), "status": st.cpu().numpy(),
+                             "S": S.cpu().numpy()})
+        _same(outs)
+        for k in ("yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status"):
+            assert np.array_equal(outs[0][k], ref[k], equal_nan=True), k
+        res[flags] = outs[0]
+    _same([res[0], res[va.FLAG_COLD_ORDER]])
+    ok = o["status"] == 0
+    assert np.array_equal(res[0]["status"], o["status"])
+    assert np.array_equal(res[0]["n_acc"][ok], o["n_acc"][ok])
+    assert np.array_equal(res[0]["yT"][:, ok], o["yT"][:, ok])

@@ -338,6 +338,47 @@ bool reuse_order(const va_handle *h, int64_t lanes)
 }
 
 // ---------------- persistent form ----------------
+// Split schedule: the instances that ran long in the previous forward (the head
+// of order_fwd, counted in counters[5]) run one wave each on the caller's
+// stream, while the persistent kernels take the rest of the batch on a second
+// stream forked from it.  va_solve runs each long instance's forward and then
+// its reverse in one kernel; va_forward and va_adjoint run the forward alone
+// and the reverse alone.
+bool split_solve(const va_handle *h)
+{
+    return h->side && reuse_order(h, h->lanes_fwd);
+}
+
+// Blocks of a persistent kernel beside the long-instance kernel: the room of
+// VA_LONG_MAX waves of its own size left free, and of the rest two thirds.  The
+// bulk of the batch still ends before the long instances, which run faster
+// beside a lighter load (measured on GLV N=10: a forward alone 2.53 ms without
+// the split, 2.28 ms with all of the rest, 2.16 ms with two thirds).
+int bulk_grid(const va_handle *h, int64_t lanes)
+{
+    const int per_block = h->ks.slots(h->block);
+    const int reserve = blocks_for((int64_t)VA_LONG_MAX * h->ks.slots(64), per_block);
+    const int g = blocks_for(std::min<int64_t>(lanes, h->pb.batch), per_block);
+    return std::max(1, (g - reserve) * 2 / 3);
+}
+
+// Fork the second stream from s.  The caller enqueues the long-instance kernel
+// on s before the persistent kernels on the second stream, so that its waves
+// are placed first.
+va_status fork_side(va_handle *h, hipStream_t s)
+{
+    VA_HIP(hipEventRecord(h->ev_fork, s));
+    VA_HIP(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    return VA_OK;
+}
+
+va_status join_side(va_handle *h, hipStream_t s)
+{
+    VA_HIP(hipEventRecord(h->ev_join, h->side));
+    VA_HIP(hipStreamWaitEvent(s, h->ev_join, 0));
+    return VA_OK;
+}
+
 va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream_t s)
 {
     const int64_t B = h->pb.batch;
@@ -356,7 +397,17 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     const bool reuse = reuse_order(h, h->lanes_fwd);
     a.order = reuse ? h->order_fwd : nullptr;
     const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
-    VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+    if (split_solve(h) && h->ks.long_fwd) {
+        a.split = h->counters + 5;
+        a.part = 1;
+        if ((st = fork_side(h, s))) return st;
+        VA_HIP(h->ks.long_fwd(a, VA_LONG_MAX / 4, 256, s));
+        VA_HIP(h->ks.fwd(a, bulk_grid(h, h->lanes_fwd), h->block,
+                         h->side));
+        if ((st = join_side(h, s))) return st;
+    } else {
+        VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+    }
     if (reuse) return order_by_length(h, 1, s);
     return VA_OK;
 }
@@ -368,10 +419,13 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->counters + 1, 0, 1, s);
     if (st) return st;
-    if ((st = order_longest_first(h, s))) return st;
+    // With the split schedule the last forward left its own instances
+    // longest-first by attempts in order_fwd, and their long head counted.
+    const bool split = split_solve(h) && h->ks.long_rev;
+    if (!split && (st = order_longest_first(h, s))) return st;
     va::RevArgs a;
     a.sp = solve_params(h);
-    a.order = h->order;
+    a.order = split ? h->order_fwd : h->order;
     a.p = p;
     a.traj = h->traj;
     a.n_acc = h->n_acc;
@@ -384,25 +438,23 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     a.mu = mu;
     a.nout = nout;
     a.counter = h->counters + 1;
+    if (split) {
+        a.split = h->counters + 5;
+        a.part = 1;
+        if ((st = fork_side(h, s))) return st;
+        VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
+        VA_HIP(h->ks.rev(a, bulk_grid(h, h->lanes_rev), h->block,
+                         h->side));
+        return join_side(h, s);
+    }
     const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
     VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     return VA_OK;
 }
 
-// va_solve with the instances that ran long in the previous solve (the head of
-// order_fwd, counted in counters[5]) one wave each, forward then reverse, on
-// the caller's stream, while the persistent kernels run the forward and then
-// the reverse of the rest of the batch on a second stream forked from it.  The
-// persistent kernels leave room on the device for the long-instance waves.
-bool split_solve(const va_handle *h)
-{
-    return h->side && reuse_order(h, h->lanes_fwd);
-}
-
 va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const double *cost_w,
                       double *J, double *dJdy0, double *dJdp, hipStream_t s)
 {
-    const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->counters, 0, 4, s);
     if (st) return st;
     va::FwdArgs fa;
@@ -435,23 +487,11 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.order = h->order_fwd;
     ra.split = h->counters + 5;
     ra.part = 1;
-    // The long-instance kernel goes on the caller's stream right after the fork
-    // point, ahead of the persistent kernels (which first wait for the fork on the
-    // second stream), so that its waves are placed first.
-    VA_HIP(hipEventRecord(h->ev_fork, s));
+    if ((st = fork_side(h, s))) return st;
     VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX / 4, 256, s));
-    VA_HIP(hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    // Each persistent kernel leaves the room of VA_LONG_MAX waves of its own
-    // size, and of the rest takes two thirds: the bulk of the batch still ends
-    // before the long instances, which run faster beside a lighter load.
-    const int per_block = h->ks.slots(h->block);
-    const int reserve = blocks_for((int64_t)VA_LONG_MAX * h->ks.slots(64), per_block);
-    const int gf = blocks_for(std::min<int64_t>(h->lanes_fwd, B), per_block);
-    const int gr = blocks_for(std::min<int64_t>(h->lanes_rev, B), per_block);
-    VA_HIP(h->ks.fwd(fa, std::max(1, (gf - reserve) * 2 / 3), h->block, h->side));
-    VA_HIP(h->ks.rev(ra, std::max(1, (gr - reserve) * 2 / 3), h->block, h->side));
-    VA_HIP(hipEventRecord(h->ev_join, h->side));
-    VA_HIP(hipStreamWaitEvent(s, h->ev_join, 0));
+    VA_HIP(h->ks.fwd(fa, bulk_grid(h, h->lanes_fwd), h->block, h->side));
+    VA_HIP(h->ks.rev(ra, bulk_grid(h, h->lanes_rev), h->block, h->side));
+    if ((st = join_side(h, s))) return st;
     return order_by_length(h, 1, s);
 }
 

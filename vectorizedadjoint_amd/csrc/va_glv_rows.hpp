@@ -625,6 +625,24 @@ __global__ void __launch_bounds__(256) k_glv_rows_long(FwdArgs fa, RevArgs ra)
     glv_rows_rev_body<NN, Tab, true>(ra, b);
 }
 
+// The same instances in a forward alone (va_forward) and in a reverse alone
+// (va_adjoint, positions [0, *split) of the order of the last forward).
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs fa)
+{
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= *fa.split || w >= fa.sp.B) return;
+    glv_rows_fwd_body<NN, Tab, true, true>(fa, fa.order[w]);
+}
+
+template <int NN, class Tab>
+__global__ void __launch_bounds__(256) k_glv_rows_long_rev(RevArgs ra)
+{
+    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= *ra.split || w >= ra.sp.B) return;
+    glv_rows_rev_body<NN, Tab, true>(ra, ra.order[w]);
+}
+
 // ---------------------------------------------------------------------------
 // Plain form in the same layout: one launch per accepted step, no refill, full
 // (t, h, y, K_0..K_{S-1}) store in [step][field][instance] order.
@@ -848,6 +866,16 @@ struct LaunchGlvRows {
         hipLaunchKernelGGL((k_glv_rows_long<NN, Tab>), dim3(grid), dim3(block), 0, s, fa, ra);
         return hipGetLastError();
     }
+    static hipError_t lng_fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_long_fwd<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
+    static hipError_t lng_rev(const RevArgs &a, int grid, int block, hipStream_t s)
+    {
+        hipLaunchKernelGGL((k_glv_rows_long_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        return hipGetLastError();
+    }
     static hipError_t pfwd(const PlainFwdArgs &a, int grid, int block, hipStream_t s)
     {
         hipLaunchKernelGGL((k_glv_rows_plain_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s,
@@ -887,7 +915,11 @@ struct LaunchGlvRows {
         k->occ_rev = &occ_rev;
         k->slots = &slots;
         k->layout = 1;
-        if constexpr (Ad) k->long_fused = &lng;
+        if constexpr (Ad) {
+            k->long_fused = &lng;
+            k->long_fwd = &lng_fwd;
+            k->long_rev = &lng_rev;
+        }
     }
 };
 

@@ -159,6 +159,9 @@ struct KernelSet {
     hipError_t (*rev)(const RevArgs &, int grid, int block, hipStream_t);
     // forward then reverse of each long-running instance, one per wave (null where absent)
     hipError_t (*long_fused)(const FwdArgs &, const RevArgs &, int grid, int block, hipStream_t);
+    // the same instances' forward alone and reverse alone (null where absent)
+    hipError_t (*long_fwd)(const FwdArgs &, int grid, int block, hipStream_t);
+    hipError_t (*long_rev)(const RevArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_fwd)(const PlainFwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_rev)(const PlainRevArgs &, int grid, int block, hipStream_t);
     int (*occ_fwd)(int block);   // resident blocks per CU of the persistent kernels
