@@ -349,17 +349,17 @@ bool split_solve(const va_handle *h)
     return h->side && reuse_order(h, h->lanes_fwd);
 }
 
-// Blocks of a persistent kernel beside the long-instance kernel: the room of
-// VA_LONG_MAX waves of its own size left free, and of the rest two thirds.  The
-// bulk of the batch still ends before the long instances, which run faster
-// beside a lighter load (measured on GLV N=10: a forward alone 2.53 ms without
-// the split, 2.28 ms with all of the rest, 2.16 ms with two thirds).
+// Blocks of a persistent kernel beside the long-instance kernel: all of its
+// grid but the room of VA_LONG_MAX waves of its own size, which the long
+// instances hold.  (GLV N=10, B=65536: a solve takes 2.46 ms so, 2.68 ms with
+// two thirds of the rest; a smaller share speeds up the long instances less
+// than it slows down the bulk.)
 int bulk_grid(const va_handle *h, int64_t lanes)
 {
     const int per_block = h->ks.slots(h->block);
     const int reserve = blocks_for((int64_t)VA_LONG_MAX * h->ks.slots(64), per_block);
     const int g = blocks_for(std::min<int64_t>(lanes, h->pb.batch), per_block);
-    return std::max(1, (g - reserve) * 2 / 3);
+    return std::max(1, g - reserve);
 }
 
 // Fork the second stream from s.  The caller enqueues the long-instance kernel

@@ -26,6 +26,8 @@
 // examples/GeneralizedLotkaVolterra/main.cpp:105-119 (system, parameter layout).
 #pragma once
 
+#include <utility>
+
 #include "va_kernels_impl.hpp"
 
 namespace va {
@@ -122,16 +124,43 @@ __device__ __forceinline__ void exchange2(double *slab_a, double *slab_b, int la
     __builtin_amdgcn_wave_barrier();
 }
 
+// 64-bit DPP move (llvm.amdgcn.update.dpp at i64; the clang builtin takes 32-bit
+// operands only).  With control ROW_NEWBCAST + j (0x150 + j) every lane of a
+// 16-lane row receives lane j of that row in one v_mov_b64_dpp.
+extern "C" __device__ long va_update_dpp_i64(long old, long src, int ctrl, int row_mask,
+                                             int bank_mask, bool bound_ctrl)
+    __asm("llvm.amdgcn.update.dpp.i64");
+
+template <int J>
+__device__ __forceinline__ double row_bcast(double x)
+{
+    return __builtin_bit_cast(
+        double, va_update_dpp_i64(0, __builtin_bit_cast(long, x), 0x150 + J, 0xF, 0xF, true));
+}
+
+template <int NN, int... J>
+__device__ __forceinline__ void bcast_row0(double x, double (&out)[NN],
+                                           std::integer_sequence<int, J...>)
+{
+    ((out[J] = row_bcast<J>(x)), ...);
+}
+
 // Group 0 of a wave (lanes 0..N-1) running alone: each lane's value reaches the
-// group through scalar registers (v_readlane) instead of an LDS round trip.
+// group without an LDS round trip -- by row broadcasts when the group lies in
+// the first 16-lane row (one instruction per value), else through scalar
+// registers (v_readlane, two per value).
 template <int NN>
 __device__ __forceinline__ void bcast_group0(double x, double (&out)[NN])
 {
-    const int lo = __double2loint(x), hi = __double2hiint(x);
+    if constexpr (NN <= 16) {
+        bcast_row0<NN>(x, out, std::make_integer_sequence<int, NN>{});
+    } else {
+        const int lo = __double2loint(x), hi = __double2hiint(x);
 #pragma unroll
-    for (int j = 0; j < NN; ++j)
-        out[j] = __hiloint2double(__builtin_amdgcn_readlane(hi, j),
-                                  __builtin_amdgcn_readlane(lo, j));
+        for (int j = 0; j < NN; ++j)
+            out[j] = __hiloint2double(__builtin_amdgcn_readlane(hi, j),
+                                      __builtin_amdgcn_readlane(lo, j));
+    }
 }
 
 template <bool OneShot, int NN>
