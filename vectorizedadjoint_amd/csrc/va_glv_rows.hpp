@@ -26,6 +26,7 @@
 // examples/GeneralizedLotkaVolterra/main.cpp:105-119 (system, parameter layout).
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "va_kernels_impl.hpp"
@@ -419,10 +420,9 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
 {
     using L = GlvRows<NN>;
     constexpr int F = L::F, S = Tab::S, P = NN + NN * NN;
-    // per wave: two step buffers of S stage-input vectors and S growth-rate
-    // vectors (64 doubles each), one vector for the adjoint products u
-    __shared__ double2 xsl2[L::WAVES][2][S][32];
-    __shared__ double gsl[L::WAVES][2][S][64];
+    // per wave: one exchange slab for a stage-input vector, one for the
+    // adjoint products u
+    __shared__ double2 xsl[L::WAVES][32];
     __shared__ double2 usl2[L::WAVES][32];
     __shared__ int gidx[L::WAVES][L::GPW];
 
@@ -431,6 +431,7 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
     if (OneShot ? grp != 0 : grp >= L::GPW) return;
     const int gbase = grp * NN;
     double *uslab = reinterpret_cast<double *>(usl2[wv]);
+    double *xslab = reinterpret_cast<double *>(xsl[wv]);
     const SolveParams sp = a.sp;
     const int64_t B = sp.B;
     const double qnan = __builtin_nan("");
@@ -438,11 +439,14 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
     // Abar: column i of dJ/dA (entries A_ki, k = 0..N-1)
     double Arow[NN], Acol[NN], Abar[NN], r = 0.0, rbar = 0.0, lam = 0.0;
     double Kn[S], Kb[S];
+    // the lane's own stage inputs X_m and growth rates g_m of the last
+    // recomputed step, in slot m or S-1-m by the parity of that step
+    double Xs[S], Gs[S];
     // row nr's (h, y_i) of this lane, loaded one iteration ahead of its use
     double h_nxt = 0.0, y_nxt = 0.0, h_back = 0.0;
     int64_t b = 0;
     int32_t o = 0, nr = -1;
-    int par = 0;  // LDS buffer holding the step being reversed
+    int par = 0;  // slot parity of the step being reversed
     bool need = true, back = false, taken = false;
     const double *rows = nullptr;
     double v[NN];
@@ -522,72 +526,72 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                 h_nxt = prev[1];
                 y_nxt = prev[2 + i];
             }
-            const int pr = par ^ 1;
-            double *gr = &gsl[wv][pr][0][0];
-            if (!back) {
+            // Stage ii of step nr goes to the register slot that stage S-1-ii of
+            // step nr + 1 frees in the same iteration: slot ii when step nr has
+            // parity 0, S-1-ii when it has parity 1 (the parity alternates from
+            // step to step, so both mappings are fixed at compile time).
+            auto step = [&](auto parity) {
+                constexpr int Q = decltype(parity)::value;
+                if (!back) {
 #pragma unroll
-                for (int m = 0; m < S; ++m) {
-                    double x = yr;
+                    for (int m = 0; m < S; ++m) {
+                        double x = yr;
 #pragma unroll
-                    for (int j = 0; j < S; ++j)
-                        if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * hr, Kn[j], x);
-                    double *xs = reinterpret_cast<double *>(xsl2[wv][pr][m]);
-                    if constexpr (OneShot) {
-                        xs[lane] = x;
-                        bcast_group0<NN>(x, v);
-                    } else {
-                        exchange<NN>(xs, lane, gbase, x, v);
+                        for (int j = 0; j < S; ++j)
+                            if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * hr, Kn[j], x);
+                        if constexpr (OneShot) bcast_group0<NN>(x, v);
+                        else exchange<NN>(xslab, lane, gbase, x, v);
+                        const double g = growth_row<NN>(Arow, r, v);
+                        const int sl = Q ? S - 1 - m : m;
+                        Xs[sl] = x;
+                        Gs[sl] = g;
+                        Kn[m] = x * g;
                     }
-                    const double g = growth_row<NN>(Arow, r, v);
-                    gr[m * 64 + lane] = g;
-                    Kn[m] = x * g;
-                }
-            } else {
-                const double *gb = &gsl[wv][par][0][0];
+                } else {
 #pragma unroll
-                for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h_back) * lam;
-                double ybar = lam;
+                    for (int m = 0; m < S; ++m) Kb[m] = (Tab::B[m] * h_back) * lam;
+                    double ybar = lam;
 #pragma unroll
-                for (int ii = 0; ii < S; ++ii) {
-                    const int m = S - 1 - ii;
-                    const double *xb = reinterpret_cast<const double *>(xsl2[wv][par][m]);
-                    const double w = Kb[m];
-                    const double xown = xb[lane];
-                    const double u = w * xown;
-                    const double wg = w * gb[m * 64 + lane];
-                    double x = yr;
+                    for (int ii = 0; ii < S; ++ii) {
+                        const int m = S - 1 - ii;
+                        const int sl = Q ? S - 1 - ii : ii;
+                        const double w = Kb[m];
+                        const double xown = Xs[sl];
+                        const double u = w * xown;
+                        const double wg = w * Gs[sl];
+                        double x = yr;
 #pragma unroll
-                    for (int j = 0; j < S; ++j)
-                        if (j < ii && Tab::A[ii][j] != 0.0) x = fma(Tab::A[ii][j] * hr, Kn[j], x);
-                    double *xs = reinterpret_cast<double *>(xsl2[wv][pr][ii]);
-                    double vu[NN];
-                    xs[lane] = x;
-                    if constexpr (OneShot) {
-                        bcast_group0<NN>(x, v);
-                        bcast_group0<NN>(u, vu);
-                    } else {
-                        uslab[lane] = u;
-                        __builtin_amdgcn_wave_barrier();
-                        gather_vec<NN>(xs, gbase, v);
-                        gather_vec<NN>(uslab, gbase, vu);
-                        __builtin_amdgcn_wave_barrier();
+                        for (int j = 0; j < S; ++j)
+                            if (j < ii && Tab::A[ii][j] != 0.0) x = fma(Tab::A[ii][j] * hr, Kn[j], x);
+                        double vu[NN];
+                        if constexpr (OneShot) {
+                            bcast_group0<NN>(x, v);
+                            bcast_group0<NN>(u, vu);
+                        } else {
+                            exchange2<NN>(xslab, uslab, lane, gbase, x, v, u, vu);
+                        }
+                        const double g = growth_row<NN>(Arow, r, v);
+                        Xs[sl] = x;
+                        Gs[sl] = g;
+                        Kn[ii] = x * g;
+                        const double xi = adjoint_col<NN>(Acol, wg, vu);
+                        rbar = rbar + u;
+                        // column i of dJ/dA: dA_ki += u_k X_i, from the gathered u-vector
+                        // and the lane's own X (no X-vector gather), one fused
+                        // multiply-add each
+#pragma unroll
+                        for (int k = 0; k < NN; ++k) Abar[k] = fma(vu[k], xown, Abar[k]);
+                        ybar = ybar + xi;
+#pragma unroll
+                        for (int j = 0; j < S; ++j)
+                            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h_back, xi, Kb[j]);
                     }
-                    const double g = growth_row<NN>(Arow, r, v);
-                    gr[ii * 64 + lane] = g;
-                    Kn[ii] = x * g;
-                    const double xi = adjoint_col<NN>(Acol, wg, vu);
-                    rbar = rbar + u;
-                    // column i of dJ/dA: dA_ki += u_k X_i, from the gathered u-vector and
-                    // the lane's own X (no X-vector gather), one fused multiply-add each
-#pragma unroll
-                    for (int k = 0; k < NN; ++k) Abar[k] = fma(vu[k], xown, Abar[k]);
-                    ybar = ybar + xi;
-#pragma unroll
-                    for (int j = 0; j < S; ++j)
-                        if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h_back, xi, Kb[j]);
+                    lam = ybar;
                 }
-                lam = ybar;
-            }
+            };
+            const int pr = par ^ 1;  // parity of step nr
+            if (pr) step(std::integral_constant<int, 1>{});
+            else step(std::integral_constant<int, 0>{});
             h_back = hr;
             par = pr;
             back = nr >= 0;
