@@ -407,10 +407,12 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 // step n, recomputed from the stored (h_n, y_n); that recompute does not
 // depend on the adjoint.  So each sweep iteration recomputes step n-1 while it
 // reverses step n: stage i of the recompute runs beside backward stage S-1-i,
-// the two share one LDS exchange round (two writes, three broadcast gathers),
-// and their dependent chains interleave.  The recomputed X-vectors and the
-// lanes' g go to LDS (two buffers per wave, alternating steps), where the
-// backward of the next iteration reads them.  The first iteration of an
+// the two share one LDS exchange round (two writes, two broadcast gathers: the
+// X-vector and the u-vector), and their dependent chains interleave.  Each
+// lane keeps its own recomputed X_m and g_m in registers until the backward of
+// the next iteration reads them: the slot that backward stage S-1-i frees is
+// where recomputed stage i goes, so the slot of stage m is m or S-1-m by the
+// parity of the step.  The first iteration of an
 // instance (or of a further seed) only recomputes.  Every value is formed by
 // the same operations in the same order as in the oracle (dJ/dA: one fused
 // multiply-add per product u_k X_i).
