@@ -765,11 +765,12 @@ va_status va_create(const va_problem *pb, va_handle **out)
                   sizeof(double) * (size_t)rev_threads * h->obs_stride_max * (ks.n + 1));
         }
     }
-    if (ks.layout == 2)
+    if (ks.layout == 2) {
         alloc((void **)&h->abar_part, sizeof(double) * (size_t)h->shared_grid * ks.n * ks.n);
         // per workgroup: waves x s stages x 64 lanes x 4 growth values
         alloc((void **)&h->shared_gscr, sizeof(double) * (size_t)h->shared_grid *
                                             ks.shared_waves * ks.s_rev * 64 * 4);
+    }
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {
