@@ -57,6 +57,12 @@ __global__ void k_copy_f64(double *dst, const double *src, int64_t n)
     if (i < n) dst[i] = src[i];
 }
 
+__global__ void k_iota_i32(int32_t *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
 // The per-instance counts and status to the caller's arrays (any may be null).
 __global__ void k_copy_counts(int32_t *n_acc, int32_t *n_rej, int32_t *status,
                               const int32_t *s_acc, const int32_t *s_rej, const int32_t *s_st,
@@ -803,11 +809,12 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (e == hipSuccess) e = hipMemset(h->n_acc, 0, sizeof(int32_t) * B);
     if (e == hipSuccess) e = hipMemset(h->n_rej, 0, sizeof(int32_t) * B);
     {  // the first forward takes instances in index order
-        std::vector<int32_t> iota((size_t)B);
-        for (int64_t b = 0; b < B; ++b) iota[(size_t)b] = (int32_t)b;
-        if (e == hipSuccess)
-            e = hipMemcpy(h->order_fwd, iota.data(), sizeof(int32_t) * (size_t)B,
-                          hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_iota_i32, dim3(blocks_for(B, 256)), dim3(256), 0, nullptr,
+                               h->order_fwd, B);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) {
             release(h);
             delete h;
