@@ -272,8 +272,8 @@ def main(argv=None) -> int:
                          "kernel that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(instances would shard across GPUs with no exchange; see DESIGN.md)",
+        print("bench.py measures one GPU: there is no multi-GPU path (the instances "
+              "would shard across GPUs with no exchange)",
               file=sys.stderr)
         return 2
 
