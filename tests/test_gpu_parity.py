@@ -122,3 +122,30 @@ def test_glv_fixed_step_rk4_rows_matches_oracle():
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=6)
     _compare(g, o, grad_tol=GRAD_TIGHT)
+
+
+def test_headline_batch_matches_oracle():
+    """The headline workload at its full size (GLV N=10, B=65536, Cash-Karp 5(4),
+    tol 1e-8), every instance against the oracle: the handle's first solve (index
+    order, no split) and its second (longest-first order; the instances that ran
+    long run one per wave beside the persistent kernels) give the same bits."""
+    torch = _torch()
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import inputs as I
+    w = I.WORKLOADS["glv10"]
+    inp = I.make_inputs(w, w.batch)
+    o = run_oracle(w, inp, max_steps=1024)
+    pb = va.Problem.from_workload(w, max_steps=1024)
+    print(f"[gpu] headline glv10 B={w.batch}, two solves", flush=True)
+    outs = []
+    with va.Solver(pb) as s:
+        y0, p = _dev(inp["y0"], torch), _dev(inp["p"], torch)
+        for _ in range(2):
+            out = s.solve(y0, p, w.cost)
+            torch.cuda.synchronize()
+            outs.append({k: v.cpu().numpy() for k, v in out.items()})
+    # instances that needed 256 or more attempts exist, so the second solve splits
+    assert int(((o["n_acc"] + o["n_rej"]) >= 256).sum()) > 0
+    _compare(outs[1], o, grad_tol=GRAD_TIGHT)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
