@@ -139,3 +139,44 @@ def test_terminal_cost_solve_over_budget_uses_checkpoints(kw):
             ck.forward(y0.clone(), p)
     for k in a:
         assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+def test_full_workload_under_budget_matches_oracle():
+    """The observed workload at its full size: Van der Pol, B=262144, mu
+    log-uniform over [0.1, 100], 50 observations over [0, 20], under the 4 GB
+    trajectory budget (the per-step store would need 15 GB, so the forward keeps
+    checkpoints and the reverse recomputes every segment).  Two solves on one
+    handle -- the second takes the first's longest-first order and starts each
+    instance's checkpoint spacing from its previous step count -- each against
+    the oracle on every instance."""
+    import os
+    B = 262144
+    torch, O, va, w, inp, t_obs, y_obs, pb = _setup(B)
+    assert w.batch == B
+    budget = int(w.extra["budget_bytes"])
+    prob = va.Problem.from_workload(w, batch=B, t1=20.0, n_obs=t_obs.size, max_steps=16384,
+                                    trajectory_budget_bytes=budget)
+    print(f"[gpu] observed full workload B={B} budget={budget}", flush=True)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    outs = []
+    with va.Solver(prob) as s:
+        args = (dev(inp["y0"]), dev(inp["p"]), dev(t_obs), dev(y_obs))
+        for _ in range(2):
+            out = s.solve_observed(*args)
+            torch.cuda.synchronize()
+            outs.append({k: v.cpu().numpy() for k, v in out.items()})
+        with pytest.raises(va.VaError, match="checkpoints"):
+            s.forward(dev(inp["y0"]), dev(inp["p"]))
+    o = O.solve_batch(pb, inp["y0"], inp["p"], 0, obs=(t_obs, y_obs),
+                      nthreads=min(16, os.cpu_count() or 1))
+    assert (o["status"] == 0).all()
+    for g in outs:
+        assert np.array_equal(g["status"], o["status"])
+        assert np.array_equal(g["n_acc"], o["n_acc"]) and np.array_equal(g["n_rej"], o["n_rej"])
+        assert np.array_equal(g["yT"], o["yT"])
+        assert np.array_equal(g["J"], o["J"])
+        e0, ep = rel_err(g["dJdy0"], o["dJdy0"]).max(), rel_err(g["dJdp"], o["dJdp"]).max()
+        print(f"    max rel err dJdy0 {e0:.2e} dJdp {ep:.2e}", flush=True)
+        assert e0 <= TOL and ep <= TOL
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
