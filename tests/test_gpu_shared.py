@@ -80,3 +80,49 @@ def test_shared_handle_rejects_per_instance_entry_points():
         y = torch.zeros((128, 16), dtype=torch.float64, device="cuda")
         with pytest.raises(va.VaError, match="va_solve_shared"):
             s.forward(y, y)
+
+
+def test_shared_full_batch_slice_matches_oracle():
+    """The shared-A workload at its full size (GLV N=128, B=16384, Cash-Karp 5(4),
+    tol 1e-8: 1024 tiles over every CU, each workgroup taking several), with the
+    per-instance outputs of a 512-instance slice compared to the oracle (the whole
+    batch would take the oracle half a minute; the batch-summed dJ/dA is covered
+    against the whole-batch oracle by the smaller cases above)."""
+    import os
+    import torch
+    import oracle as O
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import constants as K
+    from vectorizedadjoint_amd import inputs as I
+    assert torch.cuda.is_available(), "no HIP device"
+    w = I.WORKLOADS["glv128_shared"]
+    B = w.batch
+    assert B == 16384
+    inp = I.make_inputs(w, B)
+    prob = va.Problem.from_workload(w, max_steps=512)
+    print(f"[gpu] shared GLV N=128 full batch B={B}", flush=True)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    with va.Solver(prob) as s:
+        out = s.solve_shared(dev(inp["y0"]), dev(inp["p"]),
+                             dev(inp["p_shared"].reshape(128, 128)), K.COST_SUM)
+        torch.cuda.synchronize()
+        g = {k: v.cpu().numpy() for k, v in out.items()}
+    assert np.all(np.isfinite(g["dJdA"]))
+    pb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
+                   abs_tol=w.tol, rel_tol=w.tol, max_steps=512)
+    b0, cnt = 7777, 512
+    o = O.solve_batch(pb, inp["y0"], inp["p"], w.cost, p_shared=inp["p_shared"], b_begin=b0,
+                      b_count=cnt, nthreads=min(16, os.cpu_count() or 1))
+    sl = slice(b0, b0 + cnt)
+    assert np.array_equal(g["status"][sl], o["status"][sl])
+    same = (g["n_acc"][sl] == o["n_acc"][sl]) & (g["n_rej"][sl] == o["n_rej"][sl])
+    assert same.mean() >= 0.999, same.mean()
+    ok = np.zeros(B, bool)
+    ok[sl] = (o["status"][sl] == 0) & same
+    assert ok.sum() >= 0.99 * cnt
+    assert rel_err(g["yT"][:, ok], o["yT"][:, ok]).max() <= 1e-12
+    assert rel_err(g["J"][ok], o["J"][ok]).max() <= 1e-12
+    e0 = rel_err(g["dJdy0"][:, ok], o["dJdy0"][:, ok]).max()
+    er = rel_err(g["dJdr"][:, ok], o["dJdp"][:, ok]).max()
+    print(f"    slice [{b0}, {b0 + cnt}): max rel err dJdy0 {e0:.2e} dJdr {er:.2e}", flush=True)
+    assert e0 <= TOL and er <= TOL
