@@ -166,14 +166,16 @@ def bench_observed(a, torch, va, I):
         nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
         opb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
                         abs_tol=w.tol, rel_tol=w.tol, max_steps=16384)
-        k = min(B, 2048 * nthreads)
+        # the whole batch: about 5 s of work on 16 threads
+        k = B
         tc = time.perf_counter()
-        O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), 0,
-                      obs=(t_obs, y_obs), nthreads=nthreads)
+        oc = O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), 0,
+                           obs=(t_obs, y_obs), nthreads=nthreads)
         dtc = time.perf_counter() - tc
-        cpu = {"value": k / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
-               "sample": f"first {k} of {B} instances (same inputs and observations), "
-                         f"{dtc:.1f} s wall, {nthreads} threads"}
+        k_ok = int((oc["status"] == 0).sum())
+        cpu = {"value": k_ok / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
+               "sample": f"all {k} instances (same inputs and observations; instances that "
+                         f"end OK per second), {dtc:.1f} s wall, {nthreads} threads"}
     res = {
         "metric": "observed adjoint gradients/sec, Van der Pol log-uniform mu, B=262144, "
                   "50 observations, FP64, 4 GB trajectory budget",
@@ -237,13 +239,16 @@ def bench_shared(a, torch, va, I):
         nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
         opb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
                         abs_tol=w.tol, rel_tol=w.tol, max_steps=max_steps)
-        k = min(B, 16 * nthreads)
+        # about 10 s of work: some 600 instances per second on 16 threads
+        k = min(B, 320 * nthreads)
         tc = time.perf_counter()
-        O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), w.cost,
-                      p_shared=inp["p_shared"], nthreads=nthreads)
+        oc = O.solve_batch(opb, inp["y0"][:, :k].copy(), inp["p"][:, :k].copy(), w.cost,
+                           p_shared=inp["p_shared"], nthreads=nthreads)
         dtc = time.perf_counter() - tc
-        cpu = {"value": k / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
-               "sample": f"first {k} of {B} instances, {dtc:.1f} s wall, {nthreads} threads"}
+        k_ok = int((oc["status"] == 0).sum())
+        cpu = {"value": k_ok / dtc, "unit": "gradients/s", "cores": nthreads, "kind": "port",
+               "sample": f"first {k} of {B} instances (instances that end OK per second), "
+                         f"{dtc:.1f} s wall, {nthreads} threads"}
     res = {
         "metric": "adjoint gradients/sec, GLV N=128 shared A, B=16384, FP64 MFMA",
         "value": n_ok / sec, "unit": "gradients/s", "n_gpus": 1, "steps": a.steps,
