@@ -144,7 +144,8 @@ void va_destroy(va_handle *h);
 /* runge_kutta(stepper, system, x0, p, t0, t1, dt, driver)
  * (lib/include/runge_kutta.hpp:47-59 -> detail/runge_kutta.hpp:36-118):
  * y: in y0 [N][B], out y(T) (in place, as x0 in the reference); p: [P][B].
- * Stores (t_n, y_n) before every accepted step and at the end.  n_accepted,
+ * Stores (t_n, y_n) before every accepted step and at the end, with h_n = t_{n+1} - t_n
+ * (the reference's GetDt, StateStorage.hpp:22; its reverse sweep uses it).  n_accepted,
  * n_rejected, lane_status: [B] int32, each may be NULL. */
 va_status va_forward(va_handle *h, double *y, const double *p, int32_t *n_accepted,
                      int32_t *n_rejected, int32_t *lane_status, void *stream);
@@ -203,12 +204,23 @@ va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const
 
 /* Driver::GetT/GetTime/GetDt/GetState (lib/include/Driver.hpp:53-65,
  * StateStorage.hpp:17-42): copies the rows stored for one instance by the last
- * va_forward into HOST memory, row n = (t_n, h_n, y_n[0..N-1]) with h_n the
- * accepted step taken from t_n (0 on the final row).  At most max_rows rows
+ * va_forward into HOST memory, row n = (t_n, h_n, y_n[0..N-1]) with h_n = t_{n+1} - t_n
+ * (StateStorage.hpp:22; 0 on the final row).  At most max_rows rows
  * are copied; *n_rows receives the number stored (accepted steps + 1).
  * Synchronises the stream. */
 va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
                         int64_t *n_rows, void *stream);
+
+/* Per-instance digest of the step sequence stored by the last forward (no reference
+ * counterpart: it lets a caller check that two solves -- this library's and the
+ * reference's, or two of this library's -- took the same accepted steps without
+ * copying trajectories).  digest: [B] device uint64.  For an instance whose forward
+ * ended OK: d = 0x9E3779B97F4A7C15, then for each stored point n = 0..n_accepted,
+ * d = mix(d ^ bits(t_n)), d = mix(d ^ bits(h_n)), where mix is the SplitMix64
+ * finaliser, bits the IEEE-754 bit pattern, h_n = t_{n+1} - t_n (Driver::GetDt,
+ * lib/include/StateStorage.hpp:22) and h = 0 on the final point; 0 otherwise.
+ * Not available on handles that keep checkpoints instead of a store. */
+va_status va_step_digest(va_handle *h, uint64_t *digest, void *stream);
 
 /* Largest accepted-step count of the last forward pass (synchronises the stream). */
 va_status va_max_steps_seen(va_handle *h, int32_t *out, void *stream);

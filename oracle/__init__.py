@@ -121,6 +121,12 @@ def lib():
                                       dp, dp, dp, C.c_int, dp, dp, dp, dp, dp, dp,
                                       ip, ip, ip, C.c_int]
         L.orc_solve_batch.restype = C.c_int
+        L.orc_solve_batch_digest.argtypes = [C.POINTER(Problem), C.c_int64, C.c_int64,
+                                             C.c_int64, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp,
+                                             dp, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int]
+        L.orc_solve_batch_digest.restype = C.c_int
+        L.orc_step_digest.argtypes = [C.POINTER(Trajectory)]
+        L.orc_step_digest.restype = C.c_uint64
         L.orc_forward_obs.argtypes = [C.POINTER(Problem), C.POINTER(Obs), dp, dp,
                                       C.POINTER(Trajectory), C.POINTER(Stats), ip, dp]
         L.orc_forward_obs.restype = C.c_int64
@@ -310,6 +316,11 @@ class Solve:
         lib().orc_sensitivity_matrix(C.byref(self.pb), _dp(self.p), C.byref(self.tr), _dp(S))
         return S
 
+    @property
+    def digest(self):
+        """orc_step_digest of the stored step sequence."""
+        return int(lib().orc_step_digest(C.byref(self.tr)))
+
     def replay(self, y0, p=None):
         y0 = np.ascontiguousarray(y0, dtype=np.float64)
         p = self.p if p is None else np.ascontiguousarray(p, dtype=np.float64)
@@ -334,6 +345,7 @@ def solve_batch(pb, y0, p, cost, cost_w=None, p_shared=None, b_begin=0, b_count=
         "J": np.full(B, np.nan), "yT": np.full((N, B), np.nan), "dJdy0": np.full((N, B), np.nan),
         "dJdp": np.full((P, B), np.nan), "n_acc": np.zeros(B, np.int32),
         "n_rej": np.zeros(B, np.int32), "status": np.full(B, -1, np.int32),
+        "digest": np.zeros(B, np.uint64),
     }
     if shared:
         out["dJdA"] = np.zeros(N * N)
@@ -348,11 +360,11 @@ def solve_batch(pb, y0, p, cost, cost_w=None, p_shared=None, b_begin=0, b_count=
         if rc != 0:
             raise ValueError("orc_solve_batch_obs rejected its arguments")
         return out
-    rc = lib().orc_solve_batch(
+    rc = lib().orc_solve_batch_digest(
         C.byref(pb), B, b_begin, b_count, _dp(y0), _dp(p), _dp(p_shared) if shared else None,
         cost, _dp(cw), _dp(out["J"]), _dp(out["yT"]), _dp(out["dJdy0"]), _dp(out["dJdp"]),
         _dp(out["dJdA"]) if shared else None, _ip(out["n_acc"]), _ip(out["n_rej"]),
-        _ip(out["status"]), nthreads)
+        _ip(out["status"]), out["digest"].ctypes.data_as(C.POINTER(C.c_uint64)), nthreads)
     if rc != 0:
         raise ValueError("orc_solve_batch rejected its arguments")
     return out

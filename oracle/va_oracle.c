@@ -14,6 +14,37 @@
 #define ORC_MUTANT 0
 #endif
 
+/* Operation sequence.  By default every step follows the IEEE operation order
+ * of the reference's own arithmetic: Boost.odeint 1.74's scale_sum stage sums
+ * and rel_error norm with each product rounded (the reference is built without
+ * FMA contraction: -mavx2, no -mfma, examples/GeneralizedLotkaVolterra/build/
+ * CMakeFiles/lotka.dir/flags.make:9), the example functors' loop orders, the
+ * C library's pow in the step adjuster, and the reverse sweep of
+ * lib/include/detail/backpropagation.hpp:83-158 with h = t[n+1] - t[n].
+ * ORC_FUSED=1 selects an alternative sequence -- fused multiply-adds, split
+ * growth sums, a Newton root in place of pow, the stored step size in the
+ * reverse -- kept only so that tests can measure how far results move when
+ * nothing but the rounding sequence changes. */
+#ifndef ORC_FUSED
+#define ORC_FUSED 0
+#endif
+/* ORC_FUSED_REVERSE: the alternative sequence in the reverse sweep only (stage
+ * recompute, growth rates, Kbar and parameter accumulation, stored step), with
+ * the reference's forward -- so the gradients' sensitivity to the reverse
+ * sweep's rounding alone can be measured. */
+#ifndef ORC_FUSED_REVERSE
+#define ORC_FUSED_REVERSE ORC_FUSED
+#endif
+#if ORC_FUSED
+#define ORC_MAD(a, b, c) fma((a), (b), (c))
+#define STORED_STEP(t_next, t, h) (h)
+#else
+#define ORC_MAD(a, b, c) ((c) + (a) * (b))
+/* the trajectory keeps h_n = t_{n+1} - t_n, the reference's GetDt
+ * (StateStorage.hpp:22), which its reverse uses */
+#define STORED_STEP(t_next, t, h) ((t_next) - (t))
+#endif
+
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
@@ -253,19 +284,25 @@ int orc_get_tableau(int id, orc_tableau *t)
 /* AadData.hpp:124-171,291-330).                                             */
 /* ------------------------------------------------------------------------ */
 
-/* Lotka-Volterra growth rate g_i = r_i + sum_j A_ij x_j, summed as two
- * interleaved partial sums (even j starting from r_i, odd j starting from 0)
- * added at the end; the HIP kernels use the same order. */
-static double glv_growth(int n, const double *Ai, double r, const double *x)
+/* Lotka-Volterra growth rate g_i = r_i + sum_j A_ij x_j.  Reference order
+ * (examples/GeneralizedLotkaVolterra/main.cpp:112-117): sum = 0.0, then
+ * sum += A_ij x_j for j = 0..N-1, each product rounded, then r_i + sum.
+ * ORC_FUSED: two interleaved fma chains (even j from r_i, odd j from 0). */
+static double glv_growth(int n, const double *Ai, double r, const double *x, int fused)
 {
-    double ge = r, go = 0.0;
-    for (int j = 0; j < n; j += 2) ge = fma(Ai[j], x[j], ge);
-    for (int j = 1; j < n; j += 2) go = fma(Ai[j], x[j], go);
-    return ge + go;
+    if (fused) {
+        double ge = r, go = 0.0;
+        for (int j = 0; j < n; j += 2) ge = fma(Ai[j], x[j], ge);
+        for (int j = 1; j < n; j += 2) go = fma(Ai[j], x[j], go);
+        return ge + go;
+    }
+    double sum = 0.0;
+    for (int j = 0; j < n; ++j) sum = sum + Ai[j] * x[j];
+    return r + sum;
 }
 
-void orc_rhs(int system, int n, const double *x, const double *p, double t, double K,
-             double *f)
+static void rhs_v(int system, int n, const double *x, const double *p, double t, double K,
+                  double *f, int fused)
 {
     (void)t;
     switch (system) {
@@ -285,15 +322,22 @@ void orc_rhs(int system, int n, const double *x, const double *p, double t, doub
         break;
     case ORC_SYS_LOTKA_VOLTERRA:
     case ORC_SYS_LOTKA_VOLTERRA_SHARED: /* examples/GeneralizedLotkaVolterra/main.cpp:109-118 */
-        for (int i = 0; i < n; ++i) f[i] = x[i] * glv_growth(n, p + (size_t)n * (i + 1), p[i], x);
+        for (int i = 0; i < n; ++i)
+            f[i] = x[i] * glv_growth(n, p + (size_t)n * (i + 1), p[i], x, fused);
         break;
     default:
         for (int i = 0; i < n; ++i) f[i] = NAN;
     }
 }
 
-void orc_vjp(int system, int n, int np, const double *x, const double *p, double t, double K,
-             const double *w, double *wJx, double *wJp)
+void orc_rhs(int system, int n, const double *x, const double *p, double t, double K,
+             double *f)
+{
+    rhs_v(system, n, x, p, t, K, f, ORC_FUSED);
+}
+
+static void vjp_v(int system, int n, int np, const double *x, const double *p, double t,
+                  double K, const double *w, double *wJx, double *wJp, int fused)
 {
     (void)t;
     (void)np;
@@ -331,7 +375,7 @@ void orc_vjp(int system, int n, int np, const double *x, const double *p, double
         double u[ORC_MAX_N];
         for (int i = 0; i < n; ++i) u[i] = w[i] * x[i];
         for (int j = 0; j < n; ++j) {
-            double g = glv_growth(n, p + (size_t)n * (j + 1), p[j], x);
+            double g = glv_growth(n, p + (size_t)n * (j + 1), p[j], x, fused);
             double ae = w[j] * g, ao = 0.0;
 #if ORC_MUTANT == 1 /* planted transposed operand, test builds only */
             for (int i = 0; i < n; i += 2) ae = fma(u[i], p[(size_t)n * (j + 1) + i], ae);
@@ -354,6 +398,12 @@ void orc_vjp(int system, int n, int np, const double *x, const double *p, double
     }
 }
 
+void orc_vjp(int system, int n, int np, const double *x, const double *p, double t, double K,
+             const double *w, double *wJx, double *wJp)
+{
+    vjp_v(system, n, np, x, p, t, K, w, wJx, wJp, ORC_FUSED);
+}
+
 /* ------------------------------------------------------------------------ */
 /* odeint helpers (Boost 1.74 util/detail/less_with_sign.hpp, restated)     */
 /* ------------------------------------------------------------------------ */
@@ -369,12 +419,16 @@ static int less_eq_with_sign(double t1, double t2, double dt)
     return t2 - t1 <= DBL_EPSILON;
 }
 
-/* Stage sums, identical operation sequence to the HIP kernels:
- *   X_m  = y + sum_{j<m, a_mj != 0} (a_mj h) K_j   (fma chain in j order)
- *   y'   = y + sum_{b_j != 0} (b_j h) K_j
- *   xerr = 0 + sum_{db_j != 0} (db_j h) K_j                                   */
-static void stage_input(const orc_tableau *tb, int m, int n, double h, const double *y,
-                        double K[][ORC_MAX_N], double *X)
+/* Stage sums (Boost.odeint 1.74 generic_rk_algorithm with scale_sumN, and the
+ * hand-written runge_kutta_dopri5::do_step_impl; third party, restated):
+ *   X_m  = y + (a_m0 h) K_0 + (a_m1 h) K_1 + ...    left to right, products rounded
+ *   y'   = y + (b_0 h) K_0 + ...
+ *   xerr = (db_0 h) K_0 + (db_1 h) K_1 + ...        from the first product
+ * Terms with a zero coefficient add a zero and are skipped (same value).
+ * The HIP kernels use the same sequence.  ORC_FUSED: one fma per term and
+ * xerr started from 0.                                                        */
+static void stage_input_v(const orc_tableau *tb, int m, int n, double h, const double *y,
+                          double K[][ORC_MAX_N], double *X, int fused)
 {
     for (int i = 0; i < n; ++i) X[i] = y[i];
     for (int j = 0; j < m; ++j) {
@@ -383,14 +437,20 @@ static void stage_input(const orc_tableau *tb, int m, int n, double h, const dou
         int jj = j > 0 ? j - 1 : j;
         if (a == 0.0) continue;
         double ah = a * h;
-        for (int i = 0; i < n; ++i) X[i] = fma(ah, K[jj][i], X[i]);
+        for (int i = 0; i < n; ++i) X[i] = fused ? fma(ah, K[jj][i], X[i]) : X[i] + ah * K[jj][i];
 #else
         double a = tb->a[m][j];
         if (a == 0.0) continue;
         double ah = a * h;
-        for (int i = 0; i < n; ++i) X[i] = fma(ah, K[j][i], X[i]);
+        for (int i = 0; i < n; ++i) X[i] = fused ? fma(ah, K[j][i], X[i]) : X[i] + ah * K[j][i];
 #endif
     }
+}
+
+static void stage_input(const orc_tableau *tb, int m, int n, double h, const double *y,
+                        double K[][ORC_MAX_N], double *X)
+{
+    stage_input_v(tb, m, n, h, y, K, X, ORC_FUSED);
 }
 
 typedef struct {
@@ -418,16 +478,26 @@ static int attempt(const orc_problem *pb, const orc_tableau *tb, const double *y
         double bb = tb->b[j];
         if (bb == 0.0) continue;
         double bh = bb * h;
-        for (int i = 0; i < n; ++i) w->ynew[i] = fma(bh, w->K[j][i], w->ynew[i]);
+        for (int i = 0; i < n; ++i) w->ynew[i] = ORC_MAD(bh, w->K[j][i], w->ynew[i]);
     }
     if (tb->has_err) {
+        int first = 1;
         for (int i = 0; i < n; ++i) w->xerr[i] = 0.0;
         for (int j = 0; j < tb->s_fwd; ++j) {
             double d = tb->db[j];
             if (d == 0.0) continue;
             double dh = d * h;
+#if ORC_FUSED
             for (int i = 0; i < n; ++i) w->xerr[i] = fma(dh, w->K[j][i], w->xerr[i]);
+#else
+            if (first)
+                for (int i = 0; i < n; ++i) w->xerr[i] = dh * w->K[j][i];
+            else
+                for (int i = 0; i < n; ++i) w->xerr[i] = w->xerr[i] + dh * w->K[j][i];
+#endif
+            first = 0;
         }
+        (void)first;
     }
     return evals;
 }
@@ -445,27 +515,44 @@ void orc_rk_step(const orc_problem *pb, const double *y, const double *p, double
         for (int i = 0; i < pb->n; ++i) xerr[i] = tb.has_err ? w.xerr[i] : 0.0;
 }
 
-/* default_error_checker::error (a_x = a_dxdt = 1), norm_inf with std::max
- * semantics (a NaN component never replaces the running maximum). */
+/* default_error_checker::error (a_x = a_dxdt = 1) with rel_error
+ * |xerr_i| / (atol + rtol (|y_i| + h |dydt_i|)), each operation rounded, and
+ * norm_inf with std::max semantics (a NaN component never replaces the running
+ * maximum). */
 static double error_norm(int n, double atol, double rtol, double h, const double *y,
                          const double *dydt, const double *xerr)
 {
     double m = 0.0;
     for (int i = 0; i < n; ++i) {
+#if ORC_FUSED
         double den = fma(rtol, fma(h, fabs(dydt[i]), fabs(y[i])), atol);
+#else
+        double den = atol + rtol * (fabs(y[i]) + h * fabs(dydt[i]));
+#endif
         double e = fabs(xerr[i]) / den;
         m = (m < e) ? e : m;
     }
     return m;
 }
 
+/* err^(-1/q): std::pow(err, -1.0/q) as Boost's step adjuster calls it (the C
+ * library's pow, not always correctly rounded); ORC_FUSED: the Newton root. */
+static double ctrl_pow(double err, int q)
+{
+#if ORC_FUSED
+    return orc_root(err, q);
+#else
+    return pow(err, -1.0 / (double)q);
+#endif
+}
+
 /* default_step_adjuster (max_dt = 0) */
 static double decrease_step(double h, double err, int err_order)
 {
 #if ORC_MUTANT == 5 /* planted wrong exponent, test builds only */
-    double f = 0.9 * orc_root(err, err_order);
+    double f = 0.9 * ctrl_pow(err, err_order);
 #else
-    double f = 0.9 * orc_root(err, err_order - 1); /* pow(err, -1/(err_order-1)) */
+    double f = 0.9 * ctrl_pow(err, err_order - 1); /* pow(err, -1/(err_order-1)) */
 #endif
     f = (f < 0.2) ? 0.2 : f;
     return h * f;
@@ -483,7 +570,7 @@ static double increase_step(double h, double err, int order)
     if (err < 0.5) {
         double lo = min_err_for_order(order);
         err = (lo < err) ? err : lo;
-        h = h * (0.9 * orc_root(err, order)); /* pow(err, -1/order) */
+        h = h * (0.9 * ctrl_pow(err, order)); /* pow(err, -1/order) */
     }
     return h;
 }
@@ -516,6 +603,30 @@ static int record(orc_trajectory *tr, int n, double t, const double *y)
     memcpy(tr->y + (size_t)tr->T * n, y, sizeof(double) * (size_t)n);
     tr->T += 1;
     return 0;
+}
+
+/* Digest of an accepted step sequence: a 64-bit mix over the bit patterns of
+ * (t_n, h_n) for every stored point n (h = 0 on the final point).  Equal step
+ * histories give equal digests; the device computes the same function
+ * (va_step_digest). */
+static uint64_t digest_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+uint64_t orc_step_digest(const orc_trajectory *tr)
+{
+    uint64_t d = 0x9E3779B97F4A7C15ull;
+    for (int64_t k = 0; k < tr->T; ++k) {
+        uint64_t bt, bh;
+        memcpy(&bt, &tr->t[k], 8);
+        memcpy(&bh, &tr->h[k], 8);
+        d = digest_mix(d ^ bt);
+        d = digest_mix(d ^ bh);
+    }
+    return d;
 }
 
 static int all_finite(int n, const double *y)
@@ -583,10 +694,11 @@ static int64_t forward_impl(const orc_problem *pb, const orc_obs *ob, double *y,
             orc_rhs(pb->system, n, y, p, t, pb->sys_const, w.K[0]);
             s0.rhs_evals += 1 + attempt(pb, &tb, y, p, t, dt, &w);
             memcpy(y, w.ynew, sizeof(double) * (size_t)n);
-            if (tr) tr->h[tr->T - 1] = dt;
             ++step;
             s0.accepted = (int32_t)step;
-            t = pb->t0 + (double)step * dt;
+            const double tn = pb->t0 + (double)step * dt;
+            if (tr) tr->h[tr->T - 1] = STORED_STEP(tn, t, dt);
+            t = tn;
         }
         if (record(tr, n, t, y) != 0) s0.status = ORC_MAX_STEPS;
         count = step;
@@ -630,8 +742,9 @@ static int64_t forward_impl(const orc_problem *pb, const orc_obs *ob, double *y,
                 clipped = 0;
                 ok = 0;
             } else {
-                if (tr) tr->h[tr->T - 1] = dt;
-                t = t + dt;
+                const double tn = t + dt;
+                if (tr) tr->h[tr->T - 1] = STORED_STEP(tn, t, dt);
+                t = tn;
                 dt = increase_step(dt, err, tb.order);
                 if (clipped && fabs(dt_keep) > fabs(dt)) dt = dt_keep;
                 memcpy(y, w.ynew, sizeof(double) * (size_t)n);
@@ -701,14 +814,15 @@ typedef struct {
     double xi[ORC_MAX_N];
 } orc_rwork;
 
-/* pbar += the stage's parameter VJP.  For Lotka-Volterra (per-instance A) the
- * entries A_ij get u_i x_j with one fused multiply-add each (wjp holds u_i in
- * its first n entries), the accumulation the HIP kernels use; every other
- * parameter adds its rounded VJP term. */
+/* pbar += the stage's parameter VJP, term by term (alphabar[k] += wJalpha[k],
+ * detail/backpropagation.hpp:220-223: each VJP entry is a rounded value).
+ * ORC_FUSED: for Lotka-Volterra with per-instance A the entries A_ij get
+ * u_i x_j with one fused multiply-add each (wjp holds u_i in its first n
+ * entries). */
 static void accumulate_param_bar(int system, int n, int np, const double *x, const double *wjp,
                                  double *pbar)
 {
-    if (system == ORC_SYS_LOTKA_VOLTERRA) {
+    if (ORC_FUSED_REVERSE && system == ORC_SYS_LOTKA_VOLTERRA) {
         for (int i = 0; i < n; ++i) {
             pbar[i] += wjp[i];
             double *Ab = pbar + (size_t)n * (i + 1);
@@ -725,16 +839,21 @@ static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const d
 {
     int n = pb->n, np = pb->np, s = tb->s;
     double tn = tr->t[nidx];
-    /* The reference takes h = t[n+1] - t[n] (StateStorage.hpp:22), which can differ
-     * from the accepted step by half an ulp of t; the exact accepted h is stored
-     * instead, so this is the exact derivative of the forward map. */
+    /* The reference takes h = t[n+1] - t[n] (Driver::GetDt, StateStorage.hpp:22),
+     * which can differ from the accepted step by an ulp of t.  ORC_FUSED takes
+     * the stored accepted step instead. */
+#if ORC_FUSED
     double h = tr->h[nidx];
+#else
+    double h = tr->t[nidx + 1] - tr->t[nidx];
+#endif
     const double *yn = tr->y + (size_t)nidx * n;
 
     /* compute_intermediate_states (:24-64): recompute K_m from y_n */
     for (int m = 0; m < s; ++m) {
-        stage_input(tb, m, n, h, yn, w->K, w->X[m]);
-        orc_rhs(pb->system, n, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->K[m]);
+        stage_input_v(tb, m, n, h, yn, w->K, w->X[m], ORC_FUSED_REVERSE);
+        rhs_v(pb->system, n, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->K[m],
+              ORC_FUSED_REVERSE);
     }
     /* seed (:104-112): ybar = lambda_{n+1}; Kbar_m = (b_m h) lambda_{n+1} */
     for (int i = 0; i < n; ++i) w->ybar[i] = lambda[i];
@@ -742,19 +861,22 @@ static void back_prop_step(const orc_problem *pb, const orc_tableau *tb, const d
         double bh = tb->b[m] * h;
         for (int i = 0; i < n; ++i) w->Kb[m][i] = bh * lambda[i];
     }
-    /* stages s..1 (:126-153) */
+    /* stages s..1 (:126-153): ybar += xi; Kbar_j += (xi a_mj) h, in that
+     * order of products (:210-216) */
     for (int m = s - 1; m >= 0; --m) {
-        orc_vjp(pb->system, n, np, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->Kb[m], w->xi,
-                wjp);
+        vjp_v(pb->system, n, np, w->X[m], p, tn + tb->c[m] * h, pb->sys_const, w->Kb[m], w->xi,
+              wjp, ORC_FUSED_REVERSE);
         for (int i = 0; i < n; ++i) w->ybar[i] += w->xi[i];
         for (int j = 0; j < m; ++j) {
             double a = tb->a[m][j];
             if (a == 0.0) continue;
-            double ah = a * h;
 #if ORC_MUTANT == 2 /* planted dropped term, test builds only */
-            (void)ah;
-#else
+            (void)a;
+#elif ORC_FUSED_REVERSE
+            double ah = a * h;
             for (int i = 0; i < n; ++i) w->Kb[j][i] = fma(ah, w->xi[i], w->Kb[j][i]);
+#else
+            for (int i = 0; i < n; ++i) w->Kb[j][i] = w->Kb[j][i] + (w->xi[i] * a) * h;
 #endif
         }
         accumulate_param_bar(pb->system, n, np, w->X[m], wjp, pbar);
@@ -848,6 +970,7 @@ typedef struct {
     const double *cost_w;
     double *J, *yT, *dJdy0, *dJdp;
     int32_t *n_acc, *n_rej, *status;
+    uint64_t *digest;
     int64_t next; /* shared work counter */
     pthread_mutex_t mtx;
     double *shared_acc;   /* sum of dJ/dA over instances */
@@ -895,6 +1018,7 @@ static void *batch_worker(void *arg)
             if (c->n_acc) c->n_acc[b] = st.accepted;
             if (c->n_rej) c->n_rej[b] = st.rejected;
             if (c->status) c->status[b] = st.status;
+            if (c->digest) c->digest[b] = orc_step_digest(&tr);
             int okf = st.status == ORC_OK;
             double Jv = 0.0;
             for (int i = 0; i < n && !c->ob; ++i) {
@@ -943,7 +1067,8 @@ static int solve_batch_impl(const orc_problem *pb, const orc_obs *ob, int64_t B,
                             int64_t b_count, const double *y0, const double *p,
                             const double *p_shared, int cost, const double *cost_w, double *J,
                             double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
-                            int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads);
+                            int32_t *n_acc, int32_t *n_rej, int32_t *status, uint64_t *digest,
+                            int nthreads);
 
 int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b_count,
                     const double *y0, const double *p, const double *p_shared,
@@ -952,7 +1077,17 @@ int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b
                     int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads)
 {
     return solve_batch_impl(pb, NULL, B, b_begin, b_count, y0, p, p_shared, cost, cost_w, J, yT,
-                            dJdy0, dJdp, dJdp_shared, n_acc, n_rej, status, nthreads);
+                            dJdy0, dJdp, dJdp_shared, n_acc, n_rej, status, NULL, nthreads);
+}
+
+int orc_solve_batch_digest(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b_count,
+                           const double *y0, const double *p, const double *p_shared, int cost,
+                           const double *cost_w, double *J, double *yT, double *dJdy0,
+                           double *dJdp, double *dJdp_shared, int32_t *n_acc, int32_t *n_rej,
+                           int32_t *status, uint64_t *digest, int nthreads)
+{
+    return solve_batch_impl(pb, NULL, B, b_begin, b_count, y0, p, p_shared, cost, cost_w, J, yT,
+                            dJdy0, dJdp, dJdp_shared, n_acc, n_rej, status, digest, nthreads);
 }
 
 int orc_solve_batch_obs(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
@@ -964,14 +1099,15 @@ int orc_solve_batch_obs(const orc_problem *pb, const orc_obs *ob, int64_t B, int
         pb->system == ORC_SYS_LOTKA_VOLTERRA_SHARED)
         return -1;
     return solve_batch_impl(pb, ob, B, b_begin, b_count, y0, p, NULL, 0, NULL, J, yT, dJdy0, dJdp,
-                            NULL, n_acc, n_rej, status, nthreads);
+                            NULL, n_acc, n_rej, status, NULL, nthreads);
 }
 
 static int solve_batch_impl(const orc_problem *pb, const orc_obs *ob, int64_t B, int64_t b_begin,
                             int64_t b_count, const double *y0, const double *p,
                             const double *p_shared, int cost, const double *cost_w, double *J,
                             double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
-                            int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads)
+                            int32_t *n_acc, int32_t *n_rej, int32_t *status, uint64_t *digest,
+                            int nthreads)
 {
     if (!pb || pb->n <= 0 || pb->n > ORC_MAX_N || b_begin < 0 || b_begin + b_count > B) return -1;
     if (!ob && cost == ORC_COST_LINEAR && !cost_w) return -1;
@@ -995,6 +1131,7 @@ static int solve_batch_impl(const orc_problem *pb, const orc_obs *ob, int64_t B,
     c.n_acc = n_acc;
     c.n_rej = n_rej;
     c.status = status;
+    c.digest = digest;
     c.next = b_begin;
     c.np_full = shared ? pb->n + pb->n * pb->n : pb->np;
     c.ob = ob;

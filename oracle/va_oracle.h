@@ -127,7 +127,7 @@ typedef struct orc_trajectory {
     int64_t cap;        /* capacity in stored points */
     int64_t T;          /* stored points = accepted steps + 1 */
     double *t;          /* [cap]   t_n                                   */
-    double *h;          /* [cap]   accepted step taken from t_n (0 at the end) */
+    double *h;          /* [cap]   h_n = t_{n+1} - t_n (0 at the end)          */
     double *y;          /* [cap*N] y_n                                   */
 } orc_trajectory;
 
@@ -169,7 +169,7 @@ void orc_rk_step(const orc_problem *pb, const double *y, const double *p, double
 int orc_traj_alloc(orc_trajectory *tr, int64_t cap, int n);
 void orc_traj_free(orc_trajectory *tr);
 
-/* Forward solve of one instance.  y (in: y0, out: y(T)); records (t_n, y_n)
+/* Forward solve of one instance.  y (in: y0, out: y(T)); records (t_n, h_n, y_n)
  * before every accepted step and once at the end.  Returns the accepted step
  * count, fills stats.  If log != NULL, each attempt appends
  * (t, h, err, accepted) to log[4*k..] up to log_cap attempts. */
@@ -201,8 +201,10 @@ void orc_adjoint_solve(const orc_problem *pb, const double *p, const orc_traject
 void orc_sensitivity_matrix(const orc_problem *pb, const double *p, const orc_trajectory *tr,
                             double *S);
 
-/* Replay the stored step sequence (frozen t_n) from y0 without error control:
- * the discrete map whose exact derivative orc_backward computes. */
+/* Replay the stored step sequence (frozen h_n = t_{n+1} - t_n) from y0 without
+ * error control: the discrete map whose exact derivative orc_backward computes
+ * (the forward itself took the accepted steps, which can differ from h_n by
+ * an ulp of t). */
 void orc_replay(const orc_problem *pb, const double *y0, const double *p,
                 const orc_trajectory *tr, double *yT);
 
@@ -216,6 +218,19 @@ int orc_solve_batch(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b
                     int cost, const double *cost_w,
                     double *J, double *yT, double *dJdy0, double *dJdp, double *dJdp_shared,
                     int32_t *n_acc, int32_t *n_rej, int32_t *status, int nthreads);
+
+/* orc_solve_batch that also writes digest[B]: orc_step_digest of each
+ * instance's stored step sequence (0 where not written). */
+int orc_solve_batch_digest(const orc_problem *pb, int64_t B, int64_t b_begin, int64_t b_count,
+                           const double *y0, const double *p, const double *p_shared, int cost,
+                           const double *cost_w, double *J, double *yT, double *dJdy0,
+                           double *dJdp, double *dJdp_shared, int32_t *n_acc, int32_t *n_rej,
+                           int32_t *status, uint64_t *digest, int nthreads);
+
+/* 64-bit digest of a stored step sequence: d = 0x9E3779B97F4A7C15, then for
+ * n = 0..T-1: d = mix(d ^ bits(t_n)); d = mix(d ^ bits(h_n)), with mix the
+ * SplitMix64 finaliser.  Equal (t_n, h_n) histories give equal digests. */
+uint64_t orc_step_digest(const orc_trajectory *tr);
 
 /* Batched solve of the observation cost (controlled loop): J[B] = running cost,
  * dJdy0 / dJdp its gradient; the observations are shared by the batch. */

@@ -1,12 +1,15 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same
 seeded inputs.
 
-Bar (per instance): identical accepted/rejected step counts; y(T) equal to the
-oracle (the forward pass follows the oracle's IEEE operation sequence, so it
-is compared bit for bit); J, dJ/dy0, dJ/dp within a norm-wise relative error
-of 1e-10 (the stated floating-point tolerance).  The persistent kernels also
-follow the oracle's reverse sequence, so their gradients are checked at
-GRAD_TIGHT = 1e-13 as well.
+The oracle follows the reference's own operation sequence (Boost.odeint's
+unfused stage sums and error norm, the C library's pow in the step adjuster,
+the example functors' loop orders, the reverse sweep of
+detail/backpropagation.hpp with h = t[n+1] - t[n]) and the HIP kernels follow it
+operation for operation.  Bar, per instance: identical status, identical
+accepted step sequence (t_n, h_n) -- compared through va_step_digest against
+orc_step_digest -- on >= 99.9% of the instances, and on those y(T), J, dJ/dy0
+and dJ/dp equal to the oracle's bit for bit (exact=True; every case here) or
+within a norm-wise relative error of 1e-10.
 """
 import numpy as np
 import pytest
@@ -16,7 +19,6 @@ from helpers import rel_err, run_oracle
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-10
-GRAD_TIGHT = 1e-13
 
 
 def _torch():
@@ -39,25 +41,31 @@ def _solve_gpu(w, inputs, mode, **kw):
         y0 = _dev(inputs["y0"], torch)
         p = _dev(inputs["p"], torch)
         out = s.solve(y0, p, w.cost)
+        out["digest"] = s.step_digest()
         torch.cuda.synchronize()
         return {k: v.cpu().numpy() for k, v in out.items()}
 
 
-def _compare(g, o, bitwise_fwd=True, grad_tol=TOL):
+def _compare(g, o, exact=True, grad_tol=TOL):
+    """Per-instance parity; prints the fractions and largest errors it finds."""
     ok = o["status"] == 0
     assert np.array_equal(g["status"], o["status"])
-    same_steps = (g["n_acc"] == o["n_acc"]) & (g["n_rej"] == o["n_rej"])
-    frac = same_steps[ok].mean()
-    assert frac >= 0.999, f"accept/reject sequences equal on only {frac:.5f} of instances"
-    m = ok & same_steps
-    if bitwise_fwd:
-        assert np.array_equal(g["yT"][:, m], o["yT"][:, m]), "y(T) differs from the oracle"
-    e_y = rel_err(g["yT"][:, m], o["yT"][:, m]).max()
-    e_J = rel_err(g["J"][m], o["J"][m]).max()
-    e_y0 = rel_err(g["dJdy0"][:, m], o["dJdy0"][:, m]).max()
-    e_p = rel_err(g["dJdp"][:, m], o["dJdp"][:, m]).max()
-    print(f"    same steps {frac:.6f}  max rel err: yT {e_y:.2e} J {e_J:.2e} "
-          f"dJdy0 {e_y0:.2e} dJdp {e_p:.2e}", flush=True)
+    same_counts = (g["n_acc"] == o["n_acc"]) & (g["n_rej"] == o["n_rej"])
+    same = same_counts
+    if "digest" in g and "digest" in o:
+        same = same & (g["digest"].view(np.uint64) == o["digest"].view(np.uint64))
+    frac = same[ok].mean()
+    m = ok & same
+    e_y = rel_err(g["yT"][:, m], o["yT"][:, m]).max(initial=0.0)
+    e_J = rel_err(g["J"][m], o["J"][m]).max(initial=0.0)
+    e_y0 = rel_err(g["dJdy0"][:, m], o["dJdy0"][:, m]).max(initial=0.0)
+    e_p = rel_err(g["dJdp"][:, m], o["dJdp"][:, m]).max(initial=0.0)
+    print(f"    same step sequence {frac:.6f} (counts {same_counts[ok].mean():.6f})  max rel err:"
+          f" yT {e_y:.2e} J {e_J:.2e} dJdy0 {e_y0:.2e} dJdp {e_p:.2e}", flush=True)
+    assert frac >= 0.999, f"step sequences equal on only {frac:.5f} of instances"
+    if exact:
+        for k in ("yT", "J", "dJdy0", "dJdp"):
+            assert np.array_equal(g[k][..., m], o[k][..., m]), f"{k} differs from the oracle"
     assert e_y <= TOL and e_J <= TOL
     assert e_y0 <= grad_tol and e_p <= grad_tol
     assert np.all(np.isnan(g["dJdp"][:, ~ok]))
@@ -73,7 +81,7 @@ def test_persistent_matches_oracle(name, B):
     inp = I.make_inputs(w, B)
     o = run_oracle(w, inp, max_steps=4096)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096)
-    _compare(g, o, grad_tol=GRAD_TIGHT)
+    _compare(g, o)
 
 
 @pytest.mark.parametrize("name,B", CASES)
@@ -95,7 +103,7 @@ def test_refill_with_few_lanes_matches_oracle(ipw):
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, lanes=120, block_size=64,
                    instances_per_wave=ipw)
-    _compare(g, o, grad_tol=GRAD_TIGHT)
+    _compare(g, o)
 
 
 @pytest.mark.parametrize("ipw", [64, 6])
@@ -110,7 +118,7 @@ def test_glv_layouts_and_tableaus_match_oracle(ipw, tableau):
     inp = I.make_inputs(w, 600)
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=ipw)
-    _compare(g, o, grad_tol=GRAD_TIGHT)
+    _compare(g, o)
 
 
 def test_glv_fixed_step_rk4_rows_matches_oracle():
@@ -121,7 +129,7 @@ def test_glv_fixed_step_rk4_rows_matches_oracle():
     inp = I.make_inputs(w, 300)
     o = run_oracle(w, inp, max_steps=1024)
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024, instances_per_wave=6)
-    _compare(g, o, grad_tol=GRAD_TIGHT)
+    _compare(g, o)
 
 
 def test_headline_batch_matches_oracle():
@@ -142,11 +150,12 @@ def test_headline_batch_matches_oracle():
         y0, p = _dev(inp["y0"], torch), _dev(inp["p"], torch)
         for _ in range(2):
             out = s.solve(y0, p, w.cost)
+            out["digest"] = s.step_digest()
             torch.cuda.synchronize()
             outs.append({k: v.cpu().numpy() for k, v in out.items()})
     # instances that needed 256 or more attempts exist, so the second solve splits
     assert int(((o["n_acc"] + o["n_rej"]) >= 256).sum()) > 0
-    _compare(outs[1], o, grad_tol=GRAD_TIGHT)
+    _compare(outs[1], o)
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
 
@@ -193,4 +202,20 @@ def test_oscillator_full_batch():
                       nthreads=min(16, os.cpu_count() or 1))
     sl = slice(b0, b0 + cnt)
     _compare({k: v[..., sl] for k, v in g.items()}, {k: v[..., sl] for k, v in o.items()},
-             grad_tol=GRAD_TIGHT)
+             )
+
+
+@pytest.mark.parametrize("name,ipw", [("vdp", 0), ("glv10", 6)])
+def test_fixed_step_euler_matches_oracle(name, ipw):
+    """The fixed-step loop (detail/runge_kutta.hpp:36-72) with Euler (ButcherTable.hpp:
+    50-65; Boost.odeint euler: y + h f(y)) on the one-lane and the 10-lane-group layouts."""
+    import dataclasses
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT, TAB_EULER
+    t1 = {"vdp": 2.0, "glv10": 5.0}[name]
+    dt = {"vdp": 1e-3, "glv10": 1e-2}[name]
+    w = dataclasses.replace(I.WORKLOADS[name], tableau=TAB_EULER, controlled=False, dt0=dt, t1=t1)
+    inp = I.make_inputs(w, 700)
+    o = run_oracle(w, inp, max_steps=4096)
+    assert (o["status"] == 0).all() and (o["n_acc"] > 400).all()
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096, instances_per_wave=ipw)
+    _compare(g, o)

@@ -141,10 +141,16 @@ def test_zero_steps_gives_identity():
     assert np.array_equal(lam, [0.3, -0.7]) and np.array_equal(mu, [0.0])
 
 
-def test_replay_reproduces_forward_bitwise():
+def test_stored_steps_are_time_differences_and_replay_matches_forward():
+    """The trajectory keeps h_n = t_{n+1} - t_n (the reference's GetDt,
+    StateStorage.hpp:22), which the reverse uses; replaying those steps
+    reproduces the forward (which took the accepted steps, an ulp of t away)
+    to rounding level."""
     pb = O.problem(O.SYS_VAN_DER_POL, 2, 1, O.TAB_CASH_KARP54, t1=5.0)
     s = O.Solve(pb, [2.0, 0.0], [2.0])
-    assert np.array_equal(s.replay([2.0, 0.0]), s.y)
+    t, h = s.times, s.steps_h
+    assert np.array_equal(h[:-1], t[1:] - t[:-1]) and h[-1] == 0.0
+    assert np.allclose(s.replay([2.0, 0.0]), s.y, rtol=1e-13, atol=0.0)
     # and the complex replay (independent code, no fma) agrees to rounding
     tab = _tab(O.TAB_CASH_KARP54)
     times = np.concatenate([[0.0], np.cumsum(s.steps_h[:-1])])

@@ -30,7 +30,7 @@ ABI_VERSION = 2
 C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
     "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_solve_shared", "va_trajectory",
-    "va_max_steps_seen", "va_last_error",
+    "va_step_digest", "va_max_steps_seen", "va_last_error",
 )
 
 
@@ -80,13 +80,15 @@ def lib() -> C.CDLL:
         L.va_sensitivity_matrix.argtypes = [vp, vp, vp, vp]
         L.va_solve.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.va_max_steps_seen.argtypes = [vp, C.POINTER(C.c_int32), vp]
+        L.va_step_digest.argtypes = [vp, vp, vp]
         L.va_solve_observed.argtypes = [vp] * 13
         L.va_solve_shared.argtypes = [vp] * 4 + [C.c_int32] + [vp] * 10
         L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
         L.va_last_error.argtypes = []
         L.va_last_error.restype = C.c_char_p
         for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
-                  "va_solve_observed", "va_solve_shared", "va_trajectory", "va_max_steps_seen"):
+                  "va_solve_observed", "va_solve_shared", "va_trajectory", "va_max_steps_seen",
+                  "va_step_digest"):
             getattr(L, f).restype = C.c_int
         _lib_handle = L
     return _lib_handle
@@ -326,6 +328,15 @@ class Solver:
         _check(lib().va_trajectory(self._h, instance, rows.ctypes.data, n.value, C.byref(n),
                                    _stream(stream)))
         return rows[: n.value]
+
+    def step_digest(self, stream=None):
+        """Per-instance digest of the stored step sequence (t_n, h_n) of the last
+        forward, as an int64 tensor [B] holding the uint64 bits (0 where the forward
+        did not end OK); equal to the oracle's orc_step_digest."""
+        import torch
+        out = torch.empty(self.problem.batch, dtype=torch.int64, device="cuda")
+        _check(lib().va_step_digest(self._h, _ptr(out), _stream(stream)))
+        return out
 
     def max_steps_seen(self, stream=None) -> int:
         v = C.c_int32(0)

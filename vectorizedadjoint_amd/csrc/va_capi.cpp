@@ -179,6 +179,45 @@ __global__ void __launch_bounds__(256) k_order_scatter(const int32_t *n_acc,
     if (b < B) order[base[k] + r] = (int32_t)b;
 }
 
+// Digest of each instance's stored step sequence: d = 0x9E3779B97F4A7C15, then for
+// every stored point n = 0..n_acc: d = mix(d ^ bits(t_n)); d = mix(d ^ bits(h_n)),
+// mix the SplitMix64 finaliser (the oracle's orc_step_digest).  0 for instances
+// whose forward did not end OK.  plain: rows [row][field][instance], else
+// [instance][row][field] with `fields` doubles per row.
+__device__ __forceinline__ uint64_t digest_mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_step_digest(const double *traj, int64_t B, int64_t cap,
+                                                     int fields, int plain, const int32_t *n_acc,
+                                                     const int32_t *status, uint64_t *out)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    uint64_t d = 0;
+    if (status[b] == VA_LANE_OK) {
+        d = 0x9E3779B97F4A7C15ull;
+        const int64_t T = (int64_t)n_acc[b] + 1;  // <= cap + 1 rows for an OK instance
+        for (int64_t k = 0; k < T && k <= cap; ++k) {
+            double t, hs;
+            if (plain) {
+                t = traj[((size_t)k * fields) * (size_t)B + b];
+                hs = traj[((size_t)k * fields + 1) * (size_t)B + b];
+            } else {
+                const double *r = traj + ((size_t)b * (size_t)(cap + 1) + (size_t)k) * fields;
+                t = r[0];
+                hs = r[1];
+            }
+            d = digest_mix(d ^ __builtin_bit_cast(uint64_t, t));
+            d = digest_mix(d ^ __builtin_bit_cast(uint64_t, hs));
+        }
+    }
+    out[b] = d;
+}
+
 inline int blocks_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 }  // namespace
@@ -697,7 +736,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
         (double)sizeof(double) * (double)B *
         (double)(pb->mode == VA_MODE_PERSISTENT
                      ? rows_b * (ks.n + 2)
-                     : rows_b * (2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n));
+                     : rows_b * (2 + (size_t)ks.n));
     const bool over = budget > 0.0 && store > budget;
     if (pb->n_obs > 0 || (obs_capable && (over || pb->checkpoint_slots > 0))) {
         // observation handles, and terminal-cost handles whose store exceeds the
@@ -780,7 +819,7 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
     } else {
-        const size_t F = 2 + (size_t)ks.n + (size_t)ks.s_rev * ks.n;
+        const size_t F = 2 + (size_t)ks.n;  // plain rows (t, h, y)
         alloc((void **)&h->ptraj, sizeof(double) * rows * F * (size_t)B);
         alloc((void **)&h->pl_t, sizeof(double) * B);
         alloc((void **)&h->pl_dt, sizeof(double) * B);
@@ -1172,7 +1211,7 @@ va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t ma
     } else {
         double *tmp = nullptr;
         VA_HIP(hipMallocAsync((void **)&tmp, bytes, s));
-        const int fields = 2 + n + h->ks.s_rev * n;
+        const int fields = 2 + n;
         hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(ncopy * (n + 2), 256)), dim3(256), 0, s,
                            h->ptraj, instance, h->pb.batch, fields, n, ncopy, tmp);
         hipError_t e = hipMemcpyAsync(rows, tmp, bytes, hipMemcpyDeviceToHost, s);
@@ -1180,6 +1219,25 @@ va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t ma
         VA_HIP(e);
     }
     VA_HIP(hipStreamSynchronize(s));
+    return VA_OK;
+}
+
+va_status va_step_digest(va_handle *h, uint64_t *digest, void *stream)
+{
+    va_status st = check_handle(h);
+    if (st) return st;
+    if (h->obs_ckpt)
+        return fail(VA_ERR_STATE,
+                    "this handle keeps checkpoints instead of a trajectory store: no digest");
+    if (!h->have_forward) return fail(VA_ERR_STATE, "va_step_digest needs a forward first");
+    if (!digest) return fail(VA_ERR_INVALID, "null digest");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = h->pb.batch;
+    const bool plain = h->pb.mode == VA_MODE_PLAIN;
+    hipLaunchKernelGGL(k_step_digest, dim3(blocks_for(B, 256)), dim3(256), 0, s,
+                       plain ? h->ptraj : h->traj, B, (int64_t)h->pb.max_steps, h->ks.n + 2,
+                       plain ? 1 : 0, h->n_acc, h->status, digest);
+    VA_HIP(hipGetLastError());
     return VA_OK;
 }
 

@@ -1,16 +1,21 @@
 // va_device.hpp -- device building blocks of the batched RK forward/adjoint path
-// for gfx950: Butcher tableaus, the step-controller pow, the built-in systems
+// for gfx950: Butcher tableaus, the step controller, the built-in systems
 // (right-hand side and vector-Jacobian products) and the explicit RK attempt.
 //
-// Rounding sequence: all HIP sources are compiled with -ffp-contract=off and
-// every multiply-add of the step arithmetic is an explicit fma(), so the IEEE
-// operation sequence is exactly the one written here.  It is the same sequence
-// the CPU oracle uses, which makes the forward pass (step sizes, accept/reject
-// decisions, y(T)) reproducible bit for bit on the host.
+// Rounding sequence: all HIP sources are compiled with -ffp-contract=off, so
+// the IEEE operation sequence is exactly the one written here.  The forward
+// step follows the reference's own arithmetic operation for operation:
+// Boost.odeint 1.74's stage sums and error norm with every product rounded
+// (the reference is built without FMA contraction), the example functors'
+// loop orders, and the C library's pow in the step adjuster (va_pow.hpp).  So
+// step sizes, accept/reject decisions and y(T) equal the reference's bit for
+// bit.  The CPU oracle uses the same sequence.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "va_pow.hpp"
 
 namespace va {
 
@@ -119,155 +124,6 @@ struct Fehlberg78 {
 };
 
 // ---------------------------------------------------------------------------
-// Step-controller root x^(-1/q), q = 2..8 (the adjuster raises the error ratio
-// to -1/q).  Own method:
-//   x = m 2^e, m in [1,2), e = q k + j with 0 <= j < q:  x^(-1/q) = 2^-k (m 2^j)^(-1/q);
-//   start  y = P_q(m - 1.5) 2^(-j/q), P_q the degree-5 Chebyshev interpolant of
-//          m^(-1/q) on [1,2] (relative error <= 1.4e-5);
-//   refine two Newton steps for y^-q = m 2^j: y += (y/q)(1 - m 2^j y^q);
-//   scale  by 2^-k, exactly.
-// IEEE basic operations, fma and integer operations on the representation
-// only, so the GPU and the host obtain the same bits.  The dependent chain is
-// about 20 FP64 operations deep, against about 60 for exp(y log x) with an
-// IEEE division inside the log.  Within one ulp of the exact root.
-// ---------------------------------------------------------------------------
-template <int Q>
-struct RootCoef;
-template <>
-struct RootCoef<2> {
-    static constexpr double p[6] = {0x1.a20d0b2f16c1fp-1, -0x1.16b40de0b165ap-2,
-                                    0x1.1559ba6062540p-3, -0x1.33fdf4c87b433p-4,
-                                    0x1.a0f6e83f40934p-5, -0x1.f69d70fc899e8p-6};
-    static constexpr double s[2] = {0x1p+0, 0x1.6a09e667f3bcdp-1};
-};
-template <>
-struct RootCoef<3> {
-    static constexpr double p[6] = {0x1.bf46914f5b764p-1, -0x1.8d95060abdbffp-3,
-                                    0x1.5ffe431f2a20ep-4, -0x1.6cc93eb91b616p-5,
-                                    0x1.d18071c6f18d9p-6, -0x1.0e87006cbb389p-6};
-    static constexpr double s[3] = {0x1p+0, 0x1.965fea53d6e3dp-1, 0x1.428a2f98d728bp-1};
-};
-template <>
-struct RootCoef<4> {
-    static constexpr double p[6] = {0x1.cea555e5f371ap-1, -0x1.346eefc659ea0p-3,
-                                    0x1.00192f0f33fe4p-4, -0x1.ffd60027966f0p-6,
-                                    0x1.3cd1d65f2357fp-6, -0x1.6962c4875515fp-7};
-    static constexpr double s[4] = {0x1p+0, 0x1.ae89f995ad3adp-1, 0x1.6a09e667f3bcdp-1,
-                                    0x1.306fe0a31b715p-1};
-};
-template <>
-struct RootCoef<5> {
-    static constexpr double p[6] = {0x1.d81ed544cf8d8p-1, -0x1.f79979e0f7a07p-4,
-                                    0x1.918282e8ac699p-5, -0x1.884dc61ebd241p-6,
-                                    0x1.dcc723482118fp-7, -0x1.0cd24fbb9fc7fp-7};
-    static constexpr double s[5] = {0x1p+0, 0x1.bdb8cdadbe120p-1, 0x1.8406003b2ae5cp-1,
-                                    0x1.51cb453b9536cp-1, 0x1.2611186bae675p-1};
-};
-template <>
-struct RootCoef<6> {
-    static constexpr double p[6] = {0x1.de8b600813da4p-1, -0x1.a9605d403f50fp-4,
-                                    0x1.49c4eebfec5aap-5, -0x1.3d52ab9af5948p-6,
-                                    0x1.7ce3ef31baabcp-7, -0x1.aa352530fcd10p-8};
-    static constexpr double s[6] = {0x1p+0, 0x1.c823e074ec129p-1, 0x1.965fea53d6e3dp-1,
-                                    0x1.6a09e667f3bcdp-1, 0x1.428a2f98d728bp-1,
-                                    0x1.1f59ac3c7d6c0p-1};
-};
-template <>
-struct RootCoef<7> {
-    static constexpr double p[6] = {0x1.e32fbc32c3ccbp-1, -0x1.702525e04166bp-4,
-                                    0x1.179a8cef4f954p-5, -0x1.0a17a5e275aafp-6,
-                                    0x1.3c8ea867dbd19p-7, -0x1.60416b7d9015bp-8};
-    static constexpr double s[7] = {0x1p+0, 0x1.cfbb031a741a5p-1, 0x1.a402feeb9c533p-1,
-                                    0x1.7c6a1f29e2ce6p-1, 0x1.588cea3f093bep-1,
-                                    0x1.381147622f886p-1, 0x1.1aa59c4115e7dp-1};
-};
-template <>
-struct RootCoef<8> {
-    static constexpr double p[6] = {0x1.e6b29102ed063p-1, -0x1.4477b7dbbc595p-4,
-                                    0x1.e5320a8f3a376p-6, -0x1.c9e63649fd2a0p-7,
-                                    0x1.0e8b165b61159p-7, -0x1.2bcadec4f337fp-8};
-    static constexpr double s[8] = {0x1p+0, 0x1.d5818dcfba487p-1, 0x1.ae89f995ad3adp-1,
-                                    0x1.8ace5422aa0dbp-1, 0x1.6a09e667f3bcdp-1,
-                                    0x1.4bfdad5362a27p-1, 0x1.306fe0a31b715p-1,
-                                    0x1.172b83c7d517bp-1};
-};
-
-// y^Q by a fixed sequence of products
-template <int Q>
-VA_HD double root_ipow(double y)
-{
-    const double y2 = y * y;
-    if constexpr (Q == 2) return y2;
-    else if constexpr (Q == 3) return y2 * y;
-    else if constexpr (Q == 4) return y2 * y2;
-    else if constexpr (Q == 5) return (y2 * y2) * y;
-    else if constexpr (Q == 6) { const double y3 = y2 * y; return y3 * y3; }
-    else if constexpr (Q == 7) { const double y3 = y2 * y, y4 = y2 * y2; return y4 * y3; }
-    else { const double y4 = y2 * y2; return y4 * y4; }
-}
-
-// 2^(-j/Q) for 0 <= j < Q by selects (no indexed load)
-template <int Q>
-VA_HD double root_scale(int j)
-{
-    double s = RootCoef<Q>::s[0];
-#pragma unroll
-    for (int i = 1; i < Q; ++i) s = (j == i) ? RootCoef<Q>::s[i] : s;
-    return s;
-}
-
-VA_HD double bits_double(uint64_t b) { return __builtin_bit_cast(double, b); }
-
-// P_Q(u) by Estrin's scheme
-template <int Q>
-VA_HD double root_poly(double u, double u2, double u4)
-{
-    using R = RootCoef<Q>;
-    const double p01 = fma(R::p[1], u, R::p[0]);
-    const double p23 = fma(R::p[3], u, R::p[2]);
-    const double p45 = fma(R::p[5], u, R::p[4]);
-    return fma(p45, u4, fma(p23, u2, p01));
-}
-
-// x^(-1/QA) if use_a, else x^(-1/QB).  Both candidates' integer splits,
-// polynomials and scale constants are formed unconditionally and the lane's
-// one is selected (no branch, no table load), so lanes that pick different
-// orders execute one instruction stream.
-template <int QA, int QB>
-VA_HD double ctrl_root(double x, bool use_a)
-{
-    static_assert(QA >= 2 && QA <= 8 && QB >= 2 && QB <= 8, "root order out of range");
-    const bool sub = x < 2.2250738585072014e-308;  // subnormal (or <= 0: replaced below)
-    const double xs = sub ? x * 0x1p64 : x;
-    const uint64_t bits = __builtin_bit_cast(uint64_t, xs);
-    const int e = (int)(bits >> 52) - 1023 - (sub ? 64 : 0);
-    const uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
-    const int ea = e + QA * 2048, eb = e + QB * 2048;  // > 0 for every finite x > 0
-    const int ka = ea / QA - 2048, kb = eb / QB - 2048;
-    const int ja = ea - (ea / QA) * QA, jb = eb - (eb / QB) * QB;
-    const int k = use_a ? ka : kb;
-    const int j = use_a ? ja : jb;
-    const double m = bits_double(mb);
-    const double xr = bits_double(mb + ((uint64_t)(uint32_t)j << 52));  // m 2^j, exact
-    const double u = m - 1.5, u2 = u * u, u4 = u2 * u2;
-    const double ya = root_poly<QA>(u, u2, u4) * root_scale<QA>(ja);
-    const double yb = root_poly<QB>(u, u2, u4) * root_scale<QB>(jb);
-    double y = use_a ? ya : yb;
-    const double iq = use_a ? 1.0 / (double)QA : 1.0 / (double)QB;
-#pragma unroll
-    for (int it = 0; it < 2; ++it) {
-        const double yqa = root_ipow<QA>(y), yqb = root_ipow<QB>(y);
-        const double r = fma(-xr, use_a ? yqa : yqb, 1.0);
-        y = fma(y * iq, r, y);
-    }
-    double res = y * bits_double((uint64_t)(uint32_t)(1023 - k) << 52);
-    res = (x == __builtin_huge_val()) ? 0.0 : res;
-    res = (x == 0.0) ? __builtin_huge_val() : res;
-    res = (x < 0.0) ? __builtin_nan("") : res;
-    return (x != x) ? x : res;
-}
-
-// ---------------------------------------------------------------------------
 // odeint helpers and controller (Boost.odeint 1.74 default_error_checker,
 // default_step_adjuster with max_dt = 0, less_with_sign; restated).
 // ---------------------------------------------------------------------------
@@ -295,30 +151,23 @@ struct FivePow {
 
 // Accept/reject decision and the next step size (default_step_adjuster with
 // max_dt = 0):
-//   reject (err > 1):   h * max(0.9 err^(-1/(ERR_ORDER-1)), 0.2)
-//   accept, err < 0.5:  h * 0.9 max(5^-ORDER, err)^(-1/ORDER)
+//   reject (err > 1):   h * max(0.9 pow(err, -1/(ERR_ORDER-1)), 0.2)
+//   accept, err < 0.5:  h * (0.9 pow(max(5^-ORDER, err), -1/ORDER))
 //   accept otherwise:   h
-// with ONE root evaluation: its argument and order are selected first, so lanes
-// of a wave that take different decisions share one instruction stream.  When
-// every active lane of the wave takes the same decision (a wave-uniform branch)
-// only that order's root is formed; otherwise both orders' candidates are
-// formed and each lane selects its own.  The lane's value is the same either way.
+// One pow evaluation whose argument and exponent are selected per lane first,
+// so lanes of a wave that take different decisions share one instruction
+// stream.  The exponents are the rounded quotients -1.0/q, as Boost forms them;
+// pow(5.0, -q) is 1/5^q exactly rounded, which the C library returns for the
+// orders here (q <= 8).
 template <class Tab>
 __device__ __forceinline__ bool control_step(double h, double err, double &h_next)
 {
     constexpr double lo = FivePow<Tab::ORDER>::inv();
-    // fixed-step tableaus (no error estimate) never reach the controller
-    constexpr int QI = Tab::ORDER >= 2 ? Tab::ORDER : 2;
-    constexpr int QD = Tab::ERR_ORDER >= 3 ? Tab::ERR_ORDER - 1 : 2;
+    constexpr double y_inc = -1.0 / (double)Tab::ORDER;
+    constexpr double y_dec = -1.0 / (double)(Tab::ERR_ORDER - 1);
     const bool accept = !(err > 1.0);
     const double base = accept ? ((lo < err) ? err : lo) : err;
-    const uint64_t all = __builtin_amdgcn_ballot_w64(true);
-    const uint64_t acc = __builtin_amdgcn_ballot_w64(accept);
-    double root;
-    if (acc == all) root = ctrl_root<QI, QI>(base, true);
-    else if (acc == 0) root = ctrl_root<QD, QD>(base, false);
-    else root = ctrl_root<QI, QD>(base, accept);
-    const double f = 0.9 * root;
+    const double f = 0.9 * powref::pow_ctrl(base, accept ? y_inc : y_dec);
     if (!accept) {
         h_next = h * ((f < 0.2) ? 0.2 : f);
     } else {
@@ -327,12 +176,19 @@ __device__ __forceinline__ bool control_step(double h, double err, double &h_nex
     return accept;
 }
 
+// Step size stored for the reverse sweep: t_{n+1} - t_n, which the reference's
+// reverse takes as its h (Driver::GetDt, lib/include/StateStorage.hpp:22); it can
+// differ from the accepted step by an ulp of t.
+VA_HD double stored_step(double t_next, double t) { return t_next - t; }
+
 // ---------------------------------------------------------------------------
 // Systems.  rhs(p, x, f); vjp(p, x, w, wJx, pbar) accumulates
-// pbar += w^T df/dp.  Formulas follow examples/VanDerPol/main.cpp:41-42,
-// examples/HarmonicOscillator/main.cpp:27-28 and
-// examples/GeneralizedLotkaVolterra/main.cpp:109-118 (accumulation starts at
-// r_i: g_i = fma(A_i,N-1, x_N-1, ... fma(A_i0, x_0, r_i))).
+// pbar += w^T df/dp term by term, each term a rounded value (alphabar[k] +=
+// wJalpha[k], detail/backpropagation.hpp:220-223).  Right-hand sides follow the
+// example functors operation for operation: examples/VanDerPol/main.cpp:41-42,
+// examples/HarmonicOscillator/main.cpp:27-28, examples/GeneralizedLotkaVolterra/
+// main.cpp:111-118 (sum = 0; sum += A_ij x_j for j = 0..N-1; x_i (r_i + sum)).
+// The VJPs are hand-written in place of the reference's AADC tape.
 // ---------------------------------------------------------------------------
 struct VanDerPol {
     static constexpr int N = 2, P = 1;
@@ -431,16 +287,13 @@ struct LotkaVolterra {
 #pragma unroll
             for (int j = 0; j < NN; ++j) q.A[i][j] = p[(int64_t)(NN * (i + 1) + j) * B + b];
     }
-    // g_i = r_i + sum_j A_ij x_j as two interleaved partial sums (even j from
-    // r_i, odd j from 0), added at the end.
+    // g_i = r_i + sum_j A_ij x_j in the functor's order
     VA_HD static double growth(const Params &q, const double *x, int i)
     {
-        double ge = q.r[i], go = 0.0;
+        double sum = 0.0;
 #pragma unroll
-        for (int j = 0; j < NN; j += 2) ge = fma(q.A[i][j], x[j], ge);
-#pragma unroll
-        for (int j = 1; j < NN; j += 2) go = fma(q.A[i][j], x[j], go);
-        return ge + go;
+        for (int j = 0; j < NN; ++j) sum = sum + q.A[i][j] * x[j];
+        return q.r[i] + sum;
     }
     VA_HD static void rhs(const Params &q, const double *x, double *f)
     {
@@ -468,17 +321,29 @@ struct LotkaVolterra {
             pbar[i] = pbar[i] + u[i];
 #pragma unroll
             for (int j = 0; j < NN; ++j)
-                pbar[NN * (i + 1) + j] = fma(u[i], x[j], pbar[NN * (i + 1) + j]);
+                pbar[NN * (i + 1) + j] = pbar[NN * (i + 1) + j] + u[i] * x[j];
         }
     }
 };
 
 // ---------------------------------------------------------------------------
-// Explicit RK attempt from (y, K[0] = f(y)):
-//   X_m  = y + sum_{j<m, a_mj != 0} (a_mj h) K_j   (fma chain in j order)
-//   y'   = y + sum_{b_j != 0} (b_j h) K_j
-//   xerr = 0 + sum_{db_j != 0} (db_j h) K_j
+// Explicit RK attempt from (y, K[0] = f(y)), Boost.odeint 1.74's sums
+// (generic_rk_algorithm with scale_sumN, runge_kutta_dopri5::do_step_impl):
+//   X_m  = y + (a_m0 h) K_0 + (a_m1 h) K_1 + ...   left to right, products rounded
+//   y'   = y + (b_0 h) K_0 + ...
+//   xerr = (db_0 h) K_0 + (db_1 h) K_1 + ...       from the first product
+// Terms with a zero coefficient add a zero and are skipped (same value).
 // ---------------------------------------------------------------------------
+template <class Tab>
+struct ErrFirst {  // index of the first nonzero embedded-error weight
+    static constexpr int value()
+    {
+        for (int j = 0; j < Tab::SF; ++j)
+            if (Tab::DB[j] != 0.0) return j;
+        return Tab::SF;
+    }
+};
+
 template <class Sys, class Tab>
 struct RK {
     static constexpr int N = Sys::N;
@@ -493,7 +358,7 @@ struct RK {
             if (j < m && Tab::A[m][j] != 0.0) {
                 const double ah = Tab::A[m][j] * h;
 #pragma unroll
-                for (int i = 0; i < N; ++i) X[i] = fma(ah, K[j][i], X[i]);
+                for (int i = 0; i < N; ++i) X[i] = X[i] + ah * K[j][i];
             }
         }
     }
@@ -515,24 +380,27 @@ struct RK {
             if (Tab::B[j] != 0.0) {
                 const double bh = Tab::B[j] * h;
 #pragma unroll
-                for (int i = 0; i < N; ++i) ynew[i] = fma(bh, K[j][i], ynew[i]);
+                for (int i = 0; i < N; ++i) ynew[i] = ynew[i] + bh * K[j][i];
             }
         }
         if constexpr (Tab::HAS_ERR) {
+            constexpr int j0 = ErrFirst<Tab>::value();
+            const double d0 = Tab::DB[j0] * h;
 #pragma unroll
-            for (int i = 0; i < N; ++i) xerr[i] = 0.0;
+            for (int i = 0; i < N; ++i) xerr[i] = d0 * K[j0][i];
 #pragma unroll
-            for (int j = 0; j < Tab::SF; ++j) {
+            for (int j = j0 + 1; j < Tab::SF; ++j) {
                 if (Tab::DB[j] != 0.0) {
                     const double dh = Tab::DB[j] * h;
 #pragma unroll
-                    for (int i = 0; i < N; ++i) xerr[i] = fma(dh, K[j][i], xerr[i]);
+                    for (int i = 0; i < N; ++i) xerr[i] = xerr[i] + dh * K[j][i];
                 }
             }
         }
     }
 
-    // default_error_checker::error with norm_inf (std::max semantics)
+    // default_error_checker::error: rel_error |xerr_i| / (atol + rtol (|y_i| +
+    // h |dydt_i|)), each operation rounded; norm_inf with std::max semantics
     __device__ __forceinline__ static double error_norm(double atol, double rtol, double h,
                                                         const double *y, const double *dydt,
                                                         const double *xerr)
@@ -540,8 +408,8 @@ struct RK {
         double m = 0.0;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            double den = fma(rtol, fma(h, fabs(dydt[i]), fabs(y[i])), atol);
-            double e = fabs(xerr[i]) / den;
+            const double den = atol + rtol * (fabs(y[i]) + h * fabs(dydt[i]));
+            const double e = fabs(xerr[i]) / den;
             m = (m < e) ? e : m;
         }
         return m;

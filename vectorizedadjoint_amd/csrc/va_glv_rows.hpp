@@ -17,9 +17,11 @@
 // instance-step by ~N, which is what bounds the long tail of instances that
 // need ~10x the median number of steps.
 //
-// Every sum is evaluated in the same order as in the one-lane kernels and the
-// CPU oracle (forward growth: j = 0..N-1 from r_i; adjoint: k = 0..N-1 from
-// w_i g_i; accumulations in stage order), so results are bit-identical.
+// Every value is formed by the same operations in the same order as in the
+// one-lane kernels and the CPU oracle (forward: the reference's sums, growth
+// j = 0..N-1 from 0 then r_i + sum; adjoint: k = 0..N-1 from w_i g_i in two
+// interleaved fma chains; accumulations in stage order), so results are
+// bit-identical.
 //
 // Reference path: lib/include/detail/runge_kutta.hpp:74-118 (forward loop),
 // lib/include/detail/backpropagation.hpp:83-158,231-254 (reverse step/loop),
@@ -172,19 +174,17 @@ __device__ __forceinline__ void xchg(double *slab, int lane, int gbase, double x
     else exchange<NN>(slab, lane, gbase, x, out);
 }
 
-// g_i = r_i + sum_j A_ij x_j: two interleaved partial sums (even j from r_i,
-// odd j from 0) added at the end -- half the dependent-FMA depth, same order as
-// the one-lane kernels and the oracle.
+// g_i = r_i + sum_j A_ij x_j in the reference functor's order
+// (examples/GeneralizedLotkaVolterra/main.cpp:112-117): sum = 0, sum += A_ij x_j
+// for j = 0..N-1 with each product rounded, then r_i + sum.
 template <int NN>
 __device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r,
                                              const double (&x)[NN])
 {
-    double ge = r, go = 0.0;
+    double sum = 0.0;
 #pragma unroll
-    for (int j = 0; j < NN; j += 2) ge = fma(Arow[j], x[j], ge);
-#pragma unroll
-    for (int j = 1; j < NN; j += 2) go = fma(Arow[j], x[j], go);
-    return ge + go;
+    for (int j = 0; j < NN; ++j) sum = sum + Arow[j] * x[j];
+    return r + sum;
 }
 
 // xi_j = w_j g_j + sum_k u_k A_kj, split the same way.
@@ -314,20 +314,21 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 double X = y;
 #pragma unroll
                 for (int j = 0; j < Tab::SF; ++j)
-                    if (j < m && Tab::A[m][j] != 0.0) X = fma(Tab::A[m][j] * h, K[j], X);
+                    if (j < m && Tab::A[m][j] != 0.0) X = X + (Tab::A[m][j] * h) * K[j];
                 xchg<OneShot, NN>(slab, lane, gbase, X, v);
                 K[m] = X * growth_row<NN>(Arow, r, v);
             }
             ynew = y;
 #pragma unroll
             for (int j = 0; j < Tab::S; ++j)
-                if (Tab::B[j] != 0.0) ynew = fma(Tab::B[j] * h, K[j], ynew);
+                if (Tab::B[j] != 0.0) ynew = ynew + (Tab::B[j] * h) * K[j];
             if constexpr (Adaptive) {
-                xerr = 0.0;
+                constexpr int j0 = ErrFirst<Tab>::value();
+                xerr = (Tab::DB[j0] * h) * K[j0];
 #pragma unroll
-                for (int j = 0; j < Tab::SF; ++j)
-                    if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
-                const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
+                for (int j = j0 + 1; j < Tab::SF; ++j)
+                    if (Tab::DB[j] != 0.0) xerr = xerr + (Tab::DB[j] * h) * K[j];
+                const double den = sp.atol + sp.rtol * (fabs(y) + h * fabs(K[0]));
                 double k0n = 0.0, ve[NN];
                 if constexpr (kSpec) {
                     if constexpr (OneShot) {
@@ -345,8 +346,9 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 if (!control_step<Tab>(dt, err, dt_next)) {
                     ++rej;
                 } else {
-                    if (i == 0) rows[(size_t)count * F + 1] = dt;
-                    t = t + dt;
+                    const double tn = t + dt;
+                    if (i == 0) rows[(size_t)count * F + 1] = stored_step(tn, t);
+                    t = tn;
                     y = ynew;
                     if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
                     if constexpr (kSpec) K[0] = k0n;
@@ -360,11 +362,12 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                     finish = true;
                 }
             } else {
-                if (i == 0) rows[(size_t)count * F + 1] = sp.dt0;
+                const double tn = sp.t0 + (double)(count + 1) * sp.dt0;
+                if (i == 0) rows[(size_t)count * F + 1] = stored_step(tn, t);
                 y = ynew;
                 if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
                 ++count;
-                t = sp.t0 + (double)count * sp.dt0;
+                t = tn;
                 new_step = true;
             }
         }
@@ -414,8 +417,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 // where recomputed stage i goes, so the slot of stage m is m or S-1-m by the
 // parity of the step.  The first iteration of an
 // instance (or of a further seed) only recomputes.  Every value is formed by
-// the same operations in the same order as in the oracle (dJ/dA: one fused
-// multiply-add per product u_k X_i).
+// the same operations in the same order as in the oracle.
 // ---------------------------------------------------------------------------
 template <int NN, class Tab, bool OneShot>
 __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
@@ -540,7 +542,7 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                         double x = yr;
 #pragma unroll
                         for (int j = 0; j < S; ++j)
-                            if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * hr, Kn[j], x);
+                            if (j < m && Tab::A[m][j] != 0.0) x = x + (Tab::A[m][j] * hr) * Kn[j];
                         if constexpr (OneShot) bcast_group0<NN>(x, v);
                         else exchange<NN>(xslab, lane, gbase, x, v);
                         const double g = growth_row<NN>(Arow, r, v);
@@ -564,7 +566,7 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                         double x = yr;
 #pragma unroll
                         for (int j = 0; j < S; ++j)
-                            if (j < ii && Tab::A[ii][j] != 0.0) x = fma(Tab::A[ii][j] * hr, Kn[j], x);
+                            if (j < ii && Tab::A[ii][j] != 0.0) x = x + (Tab::A[ii][j] * hr) * Kn[j];
                         double vu[NN];
                         if constexpr (OneShot) {
                             bcast_group0<NN>(x, v);
@@ -579,14 +581,15 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                         const double xi = adjoint_col<NN>(Acol, wg, vu);
                         rbar = rbar + u;
                         // column i of dJ/dA: dA_ki += u_k X_i, from the gathered u-vector
-                        // and the lane's own X (no X-vector gather), one fused
-                        // multiply-add each
+                        // and the lane's own X (no X-vector gather), each product
+                        // rounded before it is added
 #pragma unroll
-                        for (int k = 0; k < NN; ++k) Abar[k] = fma(vu[k], xown, Abar[k]);
+                        for (int k = 0; k < NN; ++k) Abar[k] = Abar[k] + vu[k] * xown;
                         ybar = ybar + xi;
+                        // Kbar_j += (xi a_mj) h (detail/backpropagation.hpp:215)
 #pragma unroll
                         for (int j = 0; j < S; ++j)
-                            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h_back, xi, Kb[j]);
+                            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = Kb[j] + (xi * Tab::A[m][j]) * h_back;
                     }
                     lam = ybar;
                 }
@@ -679,15 +682,15 @@ __global__ void __launch_bounds__(256) k_glv_rows_long_rev(RevArgs ra)
 }
 
 // ---------------------------------------------------------------------------
-// Plain form in the same layout: one launch per accepted step, no refill, full
-// (t, h, y, K_0..K_{S-1}) store in [step][field][instance] order.
+// Plain form in the same layout: one launch per accepted step, no refill, every
+// row (t, h, y) stored in [step][field][instance] order; one reverse launch per
+// step, recomputing the step's stages from y_n.
 // ---------------------------------------------------------------------------
 template <int NN, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
 {
     using L = GlvRows<NN>;
-    constexpr int S = Tab::S;
-    constexpr int F = 2 + NN + S * NN;
+    constexpr int F = 2 + NN;  // plain rows (t, h, y)
     __shared__ double2 slab2[L::WAVES][32];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int grp = lane / NN, i = lane - grp * NN;
@@ -754,24 +757,26 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
             double X = y;
 #pragma unroll
             for (int j = 0; j < Tab::SF; ++j)
-                if (j < m && Tab::A[m][j] != 0.0) X = fma(Tab::A[m][j] * h, K[j], X);
+                if (j < m && Tab::A[m][j] != 0.0) X = X + (Tab::A[m][j] * h) * K[j];
             exchange<NN>(slab, lane, gbase, X, v);
             K[m] = X * growth_row<NN>(Arow, r, v);
         }
         ynew = y;
 #pragma unroll
         for (int j = 0; j < Tab::S; ++j)
-            if (Tab::B[j] != 0.0) ynew = fma(Tab::B[j] * h, K[j], ynew);
+            if (Tab::B[j] != 0.0) ynew = ynew + (Tab::B[j] * h) * K[j];
         if constexpr (!Adaptive) {
-            if (i == 0) row[(size_t)B] = sp.dt0;
-            t = sp.t0 + (double)(step + 1) * sp.dt0;
+            const double tn = sp.t0 + (double)(step + 1) * sp.dt0;
+            if (i == 0) row[(size_t)B] = stored_step(tn, t);
+            t = tn;
             break;
         } else {
-            double xerr = 0.0;
+            constexpr int j0 = ErrFirst<Tab>::value();
+            double xerr = (Tab::DB[j0] * h) * K[j0];
 #pragma unroll
-            for (int j = 0; j < Tab::SF; ++j)
-                if (Tab::DB[j] != 0.0) xerr = fma(Tab::DB[j] * h, K[j], xerr);
-            const double den = fma(sp.rtol, fma(h, fabs(K[0]), fabs(y)), sp.atol);
+            for (int j = j0 + 1; j < Tab::SF; ++j)
+                if (Tab::DB[j] != 0.0) xerr = xerr + (Tab::DB[j] * h) * K[j];
+            const double den = sp.atol + sp.rtol * (fabs(y) + h * fabs(K[0]));
             exchange<NN>(slab, lane, gbase, fabs(xerr) / den, v);
             const double err = group_max<NN>(v);
             double dt_next;
@@ -779,8 +784,9 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
             if (!ok) {
                 ++rej;
             } else {
-                if (i == 0) row[(size_t)B] = dt;
-                t = t + dt;
+                const double tn = t + dt;
+                if (i == 0) row[(size_t)B] = stored_step(tn, t);
+                t = tn;
             }
             dt = dt_next;
             ++tries;
@@ -798,8 +804,6 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
             if (ok) break;
         }
     }
-#pragma unroll
-    for (int m = 0; m < S; ++m) row[(size_t)(2 + NN + m * NN + i) * B] = K[m];
     a.y[(int64_t)i * B + b] = ynew;
     if constexpr (Tab::FSAL) a.st.k0[(int64_t)i * B + b] = K[Tab::SF - 1];
     if (i == 0) {
@@ -816,7 +820,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
 {
     using L = GlvRows<NN>;
     constexpr int S = Tab::S;
-    constexpr int F = 2 + NN + S * NN;
+    constexpr int F = 2 + NN;  // plain rows (t, h, y)
     __shared__ double2 xsl2[L::WAVES][S][32];
     __shared__ double2 usl2[L::WAVES][32];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -840,19 +844,20 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
     const double *row = a.traj + (size_t)n * F * B + b;
     const double h = row[(size_t)B];
     const double yn = row[(size_t)(2 + i) * B];
+    // recompute the stages from y_n (compute_intermediate_states,
+    // detail/backpropagation.hpp:24-64)
     double X[S], g[S], K[S], Kb[S];
-#pragma unroll
-    for (int m = 0; m < S; ++m) K[m] = row[(size_t)(2 + NN + m * NN + i) * B];
 #pragma unroll
     for (int m = 0; m < S; ++m) {
         double x = yn;
 #pragma unroll
         for (int j = 0; j < S; ++j)
-            if (j < m && Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * h, K[j], x);
+            if (j < m && Tab::A[m][j] != 0.0) x = x + (Tab::A[m][j] * h) * K[j];
         X[m] = x;
         double *xs = reinterpret_cast<double *>(xsl2[wv][m]);
         exchange<NN>(xs, lane, gbase, x, v);
         g[m] = growth_row<NN>(Arow, r, v);
+        K[m] = x * g[m];
     }
     double lam = a.lambda[(int64_t)i * B + b];
     double rbar = a.st.pbar[(int64_t)i * B + b];
@@ -871,11 +876,11 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
         rbar = rbar + u;
         gather_vec<NN>(reinterpret_cast<const double *>(xsl2[wv][m]), gbase, v);
 #pragma unroll
-        for (int j = 0; j < NN; ++j) Abar[j] = fma(u, v[j], Abar[j]);
+        for (int j = 0; j < NN; ++j) Abar[j] = Abar[j] + u * v[j];
         ybar = ybar + xi;
 #pragma unroll
         for (int j = 0; j < S; ++j)
-            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = fma(Tab::A[m][j] * h, xi, Kb[j]);
+            if (j < m && Tab::A[m][j] != 0.0) Kb[j] = Kb[j] + (xi * Tab::A[m][j]) * h;
     }
     a.lambda[(int64_t)i * B + b] = ybar;
     a.st.pbar[(int64_t)i * B + b] = rbar;

@@ -126,7 +126,7 @@ __device__ __forceinline__ double stage_x(int m, double h, double y, const doubl
 {
     double x = y;
     for (int j = 0; j < m; ++j)
-        if (Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * h, K[j][jj], x);
+        if (Tab::A[m][j] != 0.0) x = x + (Tab::A[m][j] * h) * K[j][jj];
     return x;
 }
 
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_fwd(SharedFwdArgs a)
                 for (int jj = 0; jj < 4; ++jj) {
                     double x = y[jj];
                     for (int j = 0; j < m; ++j)
-                        if (Tab::A[m][j] != 0.0) x = fma(Tab::A[m][j] * dt[jj], K[j][jj], x);
+                        if (Tab::A[m][j] != 0.0) x = x + (Tab::A[m][j] * dt[jj]) * K[j][jj];
                     X[jj] = x;
                     Xs[sw(r + 4 * jj, comp)] = x;
                 }
@@ -230,13 +230,14 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_fwd(SharedFwdArgs a)
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
                 const double h = dt[jj];
-                double yn = y[jj], xe = 0.0;
+                constexpr int j0 = ErrFirst<Tab>::value();
+                double yn = y[jj], xe = (Tab::DB[j0] * h) * K[j0][jj];
                 for (int j = 0; j < Tab::S; ++j)
-                    if (Tab::B[j] != 0.0) yn = fma(Tab::B[j] * h, K[j][jj], yn);
-                for (int j = 0; j < Tab::SF; ++j)
-                    if (Tab::DB[j] != 0.0) xe = fma(Tab::DB[j] * h, K[j][jj], xe);
+                    if (Tab::B[j] != 0.0) yn = yn + (Tab::B[j] * h) * K[j][jj];
+                for (int j = j0 + 1; j < Tab::SF; ++j)
+                    if (Tab::DB[j] != 0.0) xe = xe + (Tab::DB[j] * h) * K[j][jj];
                 ynew[jj] = yn;
-                const double den = fma(sp.rtol, fma(h, fabs(K[0][jj]), fabs(y[jj])), sp.atol);
+                const double den = sp.atol + sp.rtol * (fabs(y[jj]) + h * fabs(K[0][jj]));
                 ratio[jj] = fabs(xe) / den;
             }
             tile_max(ratio, red, w, r, c);
@@ -257,10 +258,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_fwd(SharedFwdArgs a)
                 if (!control_step<Tab>(dt[jj], ratio[jj], dt_next)) {
                     ++rej[jj];
                 } else {
+                    const double tn = t[jj] + dt[jj];
                     if (w == 0 && c == 0)
                         a.traj[(size_t)b[jj] * (size_t)(sp.cap + 1) * F + (size_t)count[jj] * F +
-                               1] = dt[jj];
-                    t[jj] = t[jj] + dt[jj];
+                               1] = stored_step(tn, t[jj]);
+                    t[jj] = tn;
                     y[jj] = ynew[jj];
                     K[0][jj] = Tab::FSAL ? K[Tab::SF - 1][jj] : k0n[jj];
                     ++count[jj];
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                     ybar[jj] = ybar[jj] + xi;
                     for (int j = 0; j < m; ++j)
                         if (Tab::A[m][j] != 0.0)
-                            Kb[j][jj] = fma(Tab::A[m][j] * h[jj], xi, Kb[j][jj]);
+                            Kb[j][jj] = Kb[j][jj] + (xi * Tab::A[m][j]) * h[jj];
                 }
             }
 #pragma unroll

@@ -8,8 +8,10 @@
 //   k_rev       persistent reverse: replays the stored rows backwards, recomputes
 //               the stages and applies the RK adjoint recursion --
 //               lib/include/detail/backpropagation.hpp:83-158,231-254,280-348.
-//   k_plain_*   the plain GPU form: one launch per accepted step, no refill, the
-//               full trajectory (t, y, stage K) stored [step][field][instance].
+//   k_plain_*   the plain GPU form: one launch per accepted step, no refill,
+//               every row (t, h, y) stored [step][field][instance]; one reverse
+//               launch per step, which recomputes the step's stages from y_n as
+//               the reference's compute_intermediate_states does.
 //
 // Every lane runs a flattened state machine (one attempt / one reverse step per
 // loop trip) so the wave stays converged on the arithmetic; the refill branch is
@@ -102,8 +104,9 @@ __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
                 if (!accept) {
                     ++rej;
                 } else {
-                    rows[(size_t)count * F + 1] = dt;
-                    t = t + dt;
+                    const double tn = t + dt;
+                    rows[(size_t)count * F + 1] = stored_step(tn, t);
+                    t = tn;
 #pragma unroll
                     for (int i = 0; i < N; ++i) y[i] = ynew[i];
                     if constexpr (Tab::FSAL) {
@@ -120,7 +123,6 @@ __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
                     finish = true;
                 }
             } else {
-                rows[(size_t)count * F + 1] = sp.dt0;
 #pragma unroll
                 for (int i = 0; i < N; ++i) y[i] = ynew[i];
                 if constexpr (Tab::FSAL) {
@@ -128,7 +130,9 @@ __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
                     for (int i = 0; i < N; ++i) K[0][i] = K[Tab::SF - 1][i];
                 }
                 ++count;
-                t = sp.t0 + (double)count * sp.dt0;
+                const double tn = sp.t0 + (double)count * sp.dt0;
+                rows[(size_t)(count - 1) * F + 1] = stored_step(tn, t);
+                t = tn;
                 new_step = true;
             }
         }
@@ -159,8 +163,7 @@ template <class Sys, class Tab>
 __device__ __forceinline__ void back_prop_step(const typename Sys::Params &q, double h,
                                                const double *yn, double *lam, double *pbar)
 {
-    // h is the exact accepted step (the reference recomputes t[n+1]-t[n],
-    // StateStorage.hpp:22, which can differ from it by half an ulp of t).
+    // h = t_{n+1} - t_n as stored by the forward (the reference's GetDt).
     constexpr int N = Sys::N, S = Tab::S;
     using R = RK<Sys, Tab>;
     double K[Tab::SF][N], X[S][N], Kb[S][N];
@@ -187,9 +190,8 @@ __device__ __forceinline__ void back_prop_step(const typename Sys::Params &q, do
 #pragma unroll
         for (int j = 0; j < S; ++j) {
             if (j < m && Tab::A[m][j] != 0.0) {
-                const double ah = Tab::A[m][j] * h;
 #pragma unroll
-                for (int i = 0; i < N; ++i) Kb[j][i] = fma(ah, xi[i], Kb[j][i]);
+                for (int i = 0; i < N; ++i) Kb[j][i] = Kb[j][i] + (xi[i] * Tab::A[m][j]) * h;
             }
         }
     }
@@ -303,8 +305,8 @@ __global__ void __launch_bounds__(256) k_rev(RevArgs a)
 template <class Sys, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
 {
-    constexpr int N = Sys::N, S = Tab::S;
-    constexpr int F = 2 + N + S * N;  // fields (t_n, h_n, y_n, K_0..K_{S-1})
+    constexpr int N = Sys::N;
+    constexpr int F = 2 + N;  // fields (t_n, h_n, y_n)
     using R = RK<Sys, Tab>;
     const SolveParams sp = a.sp;
     const int64_t B = sp.B;
@@ -363,8 +365,9 @@ __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
             if (!ok) {
                 ++rej;
             } else {
-                row[(size_t)B] = dt;
-                t = t + dt;
+                const double tn = t + dt;
+                row[(size_t)B] = stored_step(tn, t);
+                t = tn;
             }
             dt = dt_next;
             ++tries;
@@ -384,13 +387,10 @@ __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
         }
     } else {
         R::attempt(q, y, K, sp.dt0, ynew, xerr);
-        row[(size_t)B] = sp.dt0;
-        t = sp.t0 + (double)(step + 1) * sp.dt0;
+        const double tn = sp.t0 + (double)(step + 1) * sp.dt0;
+        row[(size_t)B] = stored_step(tn, t);
+        t = tn;
     }
-#pragma unroll
-    for (int m = 0; m < S; ++m)
-#pragma unroll
-        for (int i = 0; i < N; ++i) row[(size_t)(2 + N + m * N + i) * B] = K[m][i];
 #pragma unroll
     for (int i = 0; i < N; ++i) a.y[(int64_t)i * B + b] = ynew[i];
     if constexpr (Tab::FSAL) {
@@ -407,9 +407,8 @@ __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
 template <class Sys, class Tab>
 __global__ void __launch_bounds__(256) k_plain_rev(PlainRevArgs a)
 {
-    constexpr int N = Sys::N, P = Sys::P, S = Tab::S;
-    constexpr int F = 2 + N + S * N;
-    using R = RK<Sys, Tab>;
+    constexpr int N = Sys::N, P = Sys::P;
+    constexpr int F = 2 + N;
     const SolveParams sp = a.sp;
     const int64_t B = sp.B;
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -421,45 +420,16 @@ __global__ void __launch_bounds__(256) k_plain_rev(PlainRevArgs a)
     Sys::load(q, a.p, B, b, sp.kconst);
     const double *row = a.traj + (size_t)n * F * B + b;
     const double h = row[(size_t)B];
-    double yn[N], K[Tab::SF][N], X[S][N], Kb[S][N], lam[N], pbar[P];
+    double yn[N], lam[N], pbar[P];
 #pragma unroll
     for (int i = 0; i < N; ++i) yn[i] = row[(size_t)(2 + i) * B];
-#pragma unroll
-    for (int m = 0; m < S; ++m)
-#pragma unroll
-        for (int i = 0; i < N; ++i) K[m][i] = row[(size_t)(2 + N + m * N + i) * B];
-#pragma unroll
-    for (int m = 0; m < S; ++m) R::stage_input(m, h, yn, K, X[m]);
 #pragma unroll
     for (int i = 0; i < N; ++i) lam[i] = a.lambda[(int64_t)i * B + b];
 #pragma unroll
     for (int k = 0; k < P; ++k) pbar[k] = a.st.pbar[(int64_t)k * B + b];
+    back_prop_step<Sys, Tab>(q, h, yn, lam, pbar);
 #pragma unroll
-    for (int m = 0; m < S; ++m) {
-        const double bh = Tab::B[m] * h;
-#pragma unroll
-        for (int i = 0; i < N; ++i) Kb[m][i] = bh * lam[i];
-    }
-    double ybar[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) ybar[i] = lam[i];
-#pragma unroll
-    for (int m = S - 1; m >= 0; --m) {
-        double xi[N];
-        Sys::vjp(q, X[m], Kb[m], xi, pbar);
-#pragma unroll
-        for (int i = 0; i < N; ++i) ybar[i] = ybar[i] + xi[i];
-#pragma unroll
-        for (int j = 0; j < S; ++j) {
-            if (j < m && Tab::A[m][j] != 0.0) {
-                const double ah = Tab::A[m][j] * h;
-#pragma unroll
-                for (int i = 0; i < N; ++i) Kb[j][i] = fma(ah, xi[i], Kb[j][i]);
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) a.lambda[(int64_t)i * B + b] = ybar[i];
+    for (int i = 0; i < N; ++i) a.lambda[(int64_t)i * B + b] = lam[i];
 #pragma unroll
     for (int k = 0; k < P; ++k) a.st.pbar[(int64_t)k * B + b] = pbar[k];
 }

@@ -170,8 +170,9 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
                 ++rej;
                 clipped = false;
             } else {
-                if constexpr (!Ckpt) rows[(size_t)count * F + 1] = dt;
-                t = t + dt;
+                const double tn = t + dt;
+                if constexpr (!Ckpt) rows[(size_t)count * F + 1] = stored_step(tn, t);
+                t = tn;
 #pragma unroll
                 for (int i = 0; i < N; ++i) y[i] = ynew[i];
                 if constexpr (Tab::FSAL) {
@@ -217,7 +218,8 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 // Reverse
 // ---------------------------------------------------------------------------
 
-// Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg) from the checkpoint c, replaying
+// Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg, h_n = t_{n+1} - t_n) from the
+// checkpoint c, replaying
 // the forward's controlled loop with the same observation stops.  Field f of
 // row k is seg[(k (N+1) + f) ss]: the lanes' buffers are interleaved (ss = the
 // grid's thread count), so a wave's stores and loads of one field coalesce.  One trip of
@@ -272,8 +274,9 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
         double dt_next;
         const bool ok = control_step<Tab>(dt, err, dt_next);
         if (ok) {
-            r[0] = dt;
-            t = t + dt;
+            const double tn = t + dt;
+            r[0] = stored_step(tn, t);
+            t = tn;
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = ynew[i];
             if constexpr (Tab::FSAL) {
