@@ -149,6 +149,47 @@ struct FivePow {
     }
 };
 
+// The controller pow's tables in LDS (5 KB per block): every kernel that runs the
+// controller copies them in with load_ctrl_tables() at its start (a block-wide
+// barrier follows), so the two table reads on each attempt's dependent chain are
+// LDS reads instead of cache reads.
+struct CtrlTablesLds {
+    double log[128][3];
+    uint64_t exp[256];
+};
+
+__device__ __forceinline__ CtrlTablesLds &ctrl_lds()
+{
+    __shared__ CtrlTablesLds tables;
+    return tables;
+}
+
+// Call at the top of the kernel, by every thread of the block.
+__device__ __forceinline__ void load_ctrl_tables()
+{
+    CtrlTablesLds &t = ctrl_lds();
+    for (int k = threadIdx.x; k < 128 * 3; k += blockDim.x)
+        (&t.log[0][0])[k] = (&powtab::LOG[0][0])[k];
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) t.exp[k] = powtab::EXP[k];
+    __syncthreads();
+}
+
+struct LdsTables {
+    __device__ static void log_entry(int i, double &invc, double &logc, double &logctail)
+    {
+        const CtrlTablesLds &t = ctrl_lds();
+        invc = t.log[i][0];
+        logc = t.log[i][1];
+        logctail = t.log[i][2];
+    }
+    __device__ static void exp_entry(uint64_t idx, uint64_t &tail_bits, uint64_t &scale_bits)
+    {
+        const CtrlTablesLds &t = ctrl_lds();
+        tail_bits = t.exp[idx];
+        scale_bits = t.exp[idx + 1];
+    }
+};
+
 // Accept/reject decision and the next step size (default_step_adjuster with
 // max_dt = 0):
 //   reject (err > 1):   h * max(0.9 pow(err, -1/(ERR_ORDER-1)), 0.2)
@@ -156,10 +197,11 @@ struct FivePow {
 //   accept otherwise:   h
 // One pow evaluation whose argument and exponent are selected per lane first,
 // so lanes of a wave that take different decisions share one instruction
-// stream.  The exponents are the rounded quotients -1.0/q, as Boost forms them;
+// stream (tables read through PowTables: LDS copies by default, which the
+// kernel must have loaded).  The exponents are the rounded quotients -1.0/q, as Boost forms them;
 // pow(5.0, -q) is 1/5^q exactly rounded, which the C library returns for the
 // orders here (q <= 8).
-template <class Tab>
+template <class Tab, class PowTables = LdsTables>
 __device__ __forceinline__ bool control_step(double h, double err, double &h_next)
 {
     constexpr double lo = FivePow<Tab::ORDER>::inv();
@@ -167,7 +209,7 @@ __device__ __forceinline__ bool control_step(double h, double err, double &h_nex
     constexpr double y_dec = -1.0 / (double)(Tab::ERR_ORDER - 1);
     const bool accept = !(err > 1.0);
     const double base = accept ? ((lo < err) ? err : lo) : err;
-    const double f = 0.9 * powref::pow_ctrl(base, accept ? y_inc : y_dec);
+    const double f = 0.9 * powref::pow_ctrl<PowTables>(base, accept ? y_inc : y_dec);
     if (!accept) {
         h_next = h * ((f < 0.2) ? 0.2 : f);
     } else {

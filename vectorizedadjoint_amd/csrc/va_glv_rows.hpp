@@ -400,6 +400,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
 template <int NN, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 {
+    if constexpr (Adaptive) load_ctrl_tables();
     glv_rows_fwd_body<NN, Tab, Adaptive, false>(a, 0);
 }
 
@@ -653,6 +654,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 template <int NN, class Tab>
 __global__ void __launch_bounds__(256) k_glv_rows_long(FwdArgs fa, RevArgs ra)
 {
+    load_ctrl_tables();
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= *fa.split || w >= fa.sp.B) return;
     const int64_t b = fa.order[w];
@@ -668,6 +670,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_long(FwdArgs fa, RevArgs ra)
 template <int NN, class Tab>
 __global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs fa)
 {
+    load_ctrl_tables();
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= *fa.split || w >= fa.sp.B) return;
     glv_rows_fwd_body<NN, Tab, true, true>(fa, fa.order[w]);
@@ -689,6 +692,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_long_rev(RevArgs ra)
 template <int NN, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
 {
+    if constexpr (Adaptive) load_ctrl_tables();
     using L = GlvRows<NN>;
     constexpr int F = 2 + NN;  // plain rows (t, h, y)
     __shared__ double2 slab2[L::WAVES][32];

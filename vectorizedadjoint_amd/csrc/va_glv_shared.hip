@@ -255,7 +255,8 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_fwd(SharedFwdArgs a)
             for (int jj = 0; jj < 4; ++jj) {
                 if (!act[jj]) continue;
                 double dt_next;
-                if (!control_step<Tab>(dt[jj], ratio[jj], dt_next)) {
+                // the tile buffers and A fill the LDS: the pow tables stay in memory
+                if (!control_step<Tab, powref::GlobalTables>(dt[jj], ratio[jj], dt_next)) {
                     ++rej[jj];
                 } else {
                     const double tn = t[jj] + dt[jj];

@@ -40,6 +40,7 @@ __device__ __forceinline__ bool all_finite(const double *y)
 template <class Sys, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_fwd(FwdArgs a)
 {
+    if constexpr (Adaptive) load_ctrl_tables();
     constexpr int N = Sys::N;
     constexpr int F = N + 2;  // row (t_n, h_n, y_n)
     using R = RK<Sys, Tab>;
@@ -305,6 +306,7 @@ __global__ void __launch_bounds__(256) k_rev(RevArgs a)
 template <class Sys, class Tab, bool Adaptive>
 __global__ void __launch_bounds__(256) k_plain_fwd(PlainFwdArgs a)
 {
+    if constexpr (Adaptive) load_ctrl_tables();
     constexpr int N = Sys::N;
     constexpr int F = 2 + N;  // fields (t_n, h_n, y_n)
     using R = RK<Sys, Tab>;

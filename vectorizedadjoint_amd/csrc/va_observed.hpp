@@ -47,6 +47,7 @@ __device__ __forceinline__ int obs_jumps(const int32_t *orow, const double *obs_
 template <class Sys, class Tab, bool Ckpt>
 __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 {
+    load_ctrl_tables();
     constexpr int N = Sys::N;
     constexpr int F = N + 2, FC = N + 3;
     using R = RK<Sys, Tab>;
@@ -303,6 +304,7 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
 template <class Sys, class Tab, bool Ckpt>
 __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
 {
+    if constexpr (Ckpt) load_ctrl_tables();
     constexpr int N = Sys::N, P = Sys::P;
     constexpr int F = N + 2, FC = N + 3, FS = N + 1;
     const SolveParams sp = a.sp;

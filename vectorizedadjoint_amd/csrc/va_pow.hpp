@@ -72,7 +72,24 @@ constexpr double E4 = 0x1.55555cf172b91p-5;
 constexpr double E5 = 0x1.1111167a4d017p-7;
 constexpr uint64_t OFF = 0x3fe6955500000000ull;
 
+// Table access.  GlobalTables reads the constant tables; a caller may supply
+// another accessor with the same two functions (the kernels read LDS copies).
+struct GlobalTables {
+    VA_POW_HD static void log_entry(int i, double &invc, double &logc, double &logctail)
+    {
+        invc = powtab::LOG[i][0];
+        logc = powtab::LOG[i][1];
+        logctail = powtab::LOG[i][2];
+    }
+    VA_POW_HD static void exp_entry(uint64_t idx, uint64_t &tail_bits, uint64_t &scale_bits)
+    {
+        tail_bits = powtab::EXP[idx];
+        scale_bits = powtab::EXP[idx + 1];
+    }
+};
+
 // log(x) as hi + *lo for x > 0 normal (x given by its bits)
+template <class T = GlobalTables>
 VA_POW_HD double log_hilo(uint64_t ix, double *lo)
 {
     const uint64_t tmp = ix - OFF;
@@ -80,9 +97,8 @@ VA_POW_HD double log_hilo(uint64_t ix, double *lo)
     const int k = (int)((int64_t)tmp >> 52);
     const double z = as_double(ix - (tmp & (0xfffull << 52)));
     const double kd = (double)k;
-    const double invc = powtab::LOG[i][0];
-    const double logc = powtab::LOG[i][1];
-    const double logctail = powtab::LOG[i][2];
+    double invc, logc, logctail;
+    T::log_entry(i, invc, logc, logctail);
     const double r = fma(z, invc, -1.0);
     const double t1 = fma(kd, LN2_HI, logc);
     const double t2 = t1 + r;
@@ -102,6 +118,7 @@ VA_POW_HD double log_hilo(uint64_t ix, double *lo)
 }
 
 // exp(x + xtail) for |x| in [2^-54, 512)
+template <class T = GlobalTables>
 VA_POW_HD double exp_hilo(double x, double xtail)
 {
     double kd = fma(INV_LN2_128, x, SHIFT);
@@ -111,8 +128,10 @@ VA_POW_HD double exp_hilo(double x, double xtail)
     r = r + xtail;
     const uint64_t idx = 2 * (ki & 127);
     const uint64_t top = ki << 45;
-    const double tail = as_double(powtab::EXP[idx]);
-    const uint64_t sbits = powtab::EXP[idx + 1] + top;
+    uint64_t tail_bits, scale_bits;
+    T::exp_entry(idx, tail_bits, scale_bits);
+    const double tail = as_double(tail_bits);
+    const uint64_t sbits = scale_bits + top;
     const double r2 = r * r;
     const double tmp = fma(r2 * r2, fma(r, E5, E4), fma(r2, fma(r, E3, E2), tail + r));
     const double scale = as_double(sbits);
@@ -120,14 +139,15 @@ VA_POW_HD double exp_hilo(double x, double xtail)
 }
 
 // pow(x, y) on the domain stated above
+template <class T = GlobalTables>
 VA_POW_HD double pow_ctrl(double x, double y)
 {
     if (x == __builtin_huge_val()) return 0.0;
     double lo;
-    const double hi = log_hilo(as_bits(x), &lo);
+    const double hi = log_hilo<T>(as_bits(x), &lo);
     const double ehi = y * hi;
     const double elo = fma(y, lo, fma(y, hi, -ehi));
-    return exp_hilo(ehi, elo);
+    return exp_hilo<T>(ehi, elo);
 }
 
 }  // namespace powref
