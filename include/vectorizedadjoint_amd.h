@@ -211,6 +211,27 @@ va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const
 va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
                         int64_t *n_rows, void *stream);
 
+/* Scratch held by va_solve_noalloc on each device it runs on (until the process ends). */
+#define VA_NOALLOC_SCRATCH_BYTES (32ll << 20)
+
+/* va_solve without a handle and without allocating device memory.  No counterpart in
+ * the reference: it is for callers that cannot allocate -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The problem is
+ * described by pb as for va_create (persistent mode, a per-instance system, no
+ * observations, checkpoints or VA_FLAG_GRAPH); the other arguments and outputs are
+ * those of va_solve.  The trajectory rows, work counters and per-instance counts
+ * live in a library-owned device array of VA_NOALLOC_SCRATCH_BYTES per device: a
+ * problem needing more than B * (max_steps+1) * (N+2) * 8 + 12 B + 256 bytes of it
+ * is rejected with VA_ERR_OUT_OF_MEMORY (GLV N=10 with max_steps 1024: up to 340
+ * instances).  Every call on a device shares that device's array and it keeps the
+ * last call's contents: calls that overlap on one device (other streams, other
+ * threads) race and give wrong results.  Same kernels, and the same results bit for
+ * bit, as va_solve. */
+va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double *p, int32_t cost,
+                           const double *cost_weights, double *J, double *yT, double *dJdy0,
+                           double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                           int32_t *lane_status, void *stream);
+
 /* Per-instance digest of the step sequence stored by the last forward (no reference
  * counterpart: it lets a caller check that two solves -- this library's and the
  * reference's, or two of this library's -- took the same accepted steps without

@@ -30,7 +30,7 @@ ABI_VERSION = 2
 C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
     "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_solve_shared", "va_trajectory",
-    "va_step_digest", "va_max_steps_seen", "va_last_error",
+    "va_step_digest", "va_max_steps_seen", "va_last_error", "va_solve_noalloc",
 )
 
 
@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
         L.va_solve.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.va_max_steps_seen.argtypes = [vp, C.POINTER(C.c_int32), vp]
         L.va_step_digest.argtypes = [vp, vp, vp]
+        L.va_solve_noalloc.argtypes = [C.POINTER(VaProblem), vp, vp, C.c_int32] + [vp] * 9
         L.va_solve_observed.argtypes = [vp] * 13
         L.va_solve_shared.argtypes = [vp] * 4 + [C.c_int32] + [vp] * 10
         L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
@@ -88,7 +89,7 @@ def lib() -> C.CDLL:
         L.va_last_error.restype = C.c_char_p
         for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
                   "va_solve_observed", "va_solve_shared", "va_trajectory", "va_max_steps_seen",
-                  "va_step_digest"):
+                  "va_step_digest", "va_solve_noalloc"):
             getattr(L, f).restype = C.c_int
         _lib_handle = L
     return _lib_handle

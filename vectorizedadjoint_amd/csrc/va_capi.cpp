@@ -12,8 +12,9 @@
 #include <string.h>
 
 #include <algorithm>
-#include <vector>
+#include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/vectorizedadjoint_amd.h"
 #include "va_kernels.hpp"
@@ -651,11 +652,14 @@ void va_problem_init(va_problem *pb)
 
 const char *va_last_error(void) { return g_err; }
 
-va_status va_create(const va_problem *pb, va_handle **out)
+}  // extern "C"
+
+namespace {
+
+// The checks of va_create that need no device state: argument ranges, the kernel
+// set for (system, N, tableau), the parameter count.
+va_status check_problem(const va_problem *pb, va::KernelSet *ks_out)
 {
-    g_err[0] = 0;
-    if (!pb || !out) return fail(VA_ERR_INVALID, "null argument");
-    *out = nullptr;
     if (pb->abi_version != VA_ABI_VERSION)
         return fail(VA_ERR_INVALID, "abi_version %d, library implements %d", pb->abi_version,
                     VA_ABI_VERSION);
@@ -691,6 +695,21 @@ va_status va_create(const va_problem *pb, va_handle **out)
     if (pb->n_param != ks.np)
         return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
                     pb->system, pb->n_state, ks.np, pb->n_param);
+    *ks_out = ks;
+    return VA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+va_status va_create(const va_problem *pb, va_handle **out)
+{
+    g_err[0] = 0;
+    if (!pb || !out) return fail(VA_ERR_INVALID, "null argument");
+    *out = nullptr;
+    va::KernelSet ks{};
+    if (va_status st = check_problem(pb, &ks)) return st;
     if (ks.layout == 2 && (pb->mode != VA_MODE_PERSISTENT || pb->n_obs > 0))
         return fail(VA_ERR_UNSUPPORTED,
                     "shared-parameter systems run in the persistent mode without observations");
@@ -1219,6 +1238,171 @@ va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t ma
         VA_HIP(e);
     }
     VA_HIP(hipStreamSynchronize(s));
+    return VA_OK;
+}
+
+}  // extern "C"
+
+// ---------------- va_solve_noalloc ----------------
+// Scratch of the entry point that allocates nothing: trajectory rows, work
+// counters and per-instance counts.  Each device the library runs on holds its
+// own copy until the process ends.  Host code never passes this name to a
+// kernel (on the host it is a stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[VA_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int VA_NOALLOC_MAX_DEVICES = 64;
+
+struct NoallocDevice {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    unsigned char *ptr = nullptr;  // this device's copy of `scratch`
+    int cu_count = 0;
+};
+
+NoallocDevice g_noalloc[VA_NOALLOC_MAX_DEVICES];
+
+// First call on a device, with that device current: resolve its copy of the
+// scratch array and its compute-unit count.
+va_status noalloc_device(int dev, NoallocDevice **out)
+{
+    if (dev < 0 || dev >= VA_NOALLOC_MAX_DEVICES)
+        return fail(VA_ERR_UNSUPPORTED, "device %d: va_solve_noalloc serves devices 0..%d", dev,
+                    VA_NOALLOC_MAX_DEVICES - 1);
+    NoallocDevice &d = g_noalloc[dev];
+    std::call_once(d.once, [&d, dev]() {
+        void *ptr = nullptr;
+        d.err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (d.err == hipSuccess)
+            d.err = hipDeviceGetAttribute(&d.cu_count, hipDeviceAttributeMultiprocessorCount, dev);
+        if (d.err == hipSuccess) d.ptr = static_cast<unsigned char *>(ptr);
+    });
+    if (d.err != hipSuccess || !d.ptr)
+        return fail(VA_ERR_HIP, "scratch of device %d: %s", dev, hipGetErrorString(d.err));
+    *out = &d;
+    return VA_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+extern "C" {
+
+va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double *p, int32_t cost,
+                           const double *cost_weights, double *J, double *yT, double *dJdy0,
+                           double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
+                           int32_t *lane_status, void *stream)
+{
+    g_err[0] = 0;
+    if (!pb) return fail(VA_ERR_INVALID, "null problem");
+    va::KernelSet ks{};
+    if (va_status st = check_problem(pb, &ks)) return st;
+    if (pb->mode != VA_MODE_PERSISTENT || ks.layout == 2 || !ks.fwd || pb->n_obs > 0 ||
+        pb->checkpoint_slots > 0 || (pb->flags & VA_FLAG_GRAPH))
+        return fail(VA_ERR_UNSUPPORTED,
+                    "va_solve_noalloc runs the persistent terminal-cost solve of a per-instance "
+                    "system: no plain mode, shared parameters, observations, checkpoints or "
+                    "VA_FLAG_GRAPH");
+    if (!y0 || !p || !J || !yT || !dJdy0 || !dJdp)
+        return fail(VA_ERR_INVALID, "y0, p, J, yT, dJdy0, dJdp are required");
+    if (cost < VA_COST_SUM || cost > VA_COST_LINEAR)
+        return fail(VA_ERR_INVALID, "unknown cost %d", cost);
+    if (cost == VA_COST_LINEAR && !cost_weights)
+        return fail(VA_ERR_INVALID, "VA_COST_LINEAR needs cost_weights");
+    const int block = pb->block_size > 0 ? pb->block_size : 256;
+    if (block % 64 != 0 || block > 256)
+        return fail(VA_ERR_INVALID, "block_size must be a multiple of 64 and <= 256");
+    const int64_t B = pb->batch;
+    const int n = ks.n;
+    // scratch layout: 8 counters | n_acc | n_rej | status | rows [B][max_steps+1][N+2]
+    const size_t cnt_bytes = align256(sizeof(int32_t) * (size_t)B);
+    const size_t off_nacc = 256, off_nrej = off_nacc + cnt_bytes, off_stat = off_nrej + cnt_bytes;
+    const size_t off_traj = off_stat + cnt_bytes;
+    const double traj_bytes =
+        8.0 * (double)B * ((double)pb->max_steps + 1.0) * (double)(n + 2);
+    const double need = (double)off_traj + traj_bytes;
+    if (need > (double)VA_NOALLOC_SCRATCH_BYTES)
+        return fail(VA_ERR_OUT_OF_MEMORY,
+                    "va_solve_noalloc needs %.0f bytes of scratch (B * (max_steps+1) * (N+2) * "
+                    "8 bytes of rows plus 12 bytes per instance) and holds %lld: use fewer "
+                    "instances or a smaller max_steps, or va_create + va_solve",
+                    need, (long long)VA_NOALLOC_SCRATCH_BYTES);
+    if (pb->trajectory_budget_bytes > 0 && traj_bytes > (double)pb->trajectory_budget_bytes)
+        return fail(VA_ERR_OUT_OF_MEMORY,
+                    "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
+                    traj_bytes, (long long)pb->trajectory_budget_bytes);
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(VA_ERR_HIP, "no HIP device");
+    NoallocDevice *d = nullptr;
+    if (va_status st = noalloc_device(dev, &d)) return st;
+    int32_t *counters = reinterpret_cast<int32_t *>(d->ptr);
+    int32_t *nacc = reinterpret_cast<int32_t *>(d->ptr + off_nacc);
+    int32_t *nrej = reinterpret_cast<int32_t *>(d->ptr + off_nrej);
+    int32_t *stat = reinterpret_cast<int32_t *>(d->ptr + off_stat);
+    double *traj = reinterpret_cast<double *>(d->ptr + off_traj);
+
+    hipStream_t s = (hipStream_t)stream;
+    const int per_block = ks.slots(block);
+    int64_t lanes_f = pb->lanes, lanes_r = pb->lanes;
+    if (lanes_f <= 0) {
+        lanes_f = (int64_t)std::max(1, ks.occ_fwd(block)) * d->cu_count * per_block;
+        lanes_r = (int64_t)std::max(1, ks.occ_rev(block)) * d->cu_count * per_block;
+    }
+    lanes_f = std::min<int64_t>(lanes_f, B);
+    lanes_r = std::min<int64_t>(lanes_r, B);
+    // the counters start at zero: a fill kernel (no memset node under graph capture)
+    if (va_status st = fill_i32(counters, 0, 8, s)) return st;
+    if (yT != y0) {
+        hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)n * B, 256)), dim3(256), 0, s, yT,
+                           y0, (int64_t)n * B);
+        VA_HIP(hipGetLastError());
+    }
+    va::SolveParams sp;
+    sp.B = B;
+    sp.cap = pb->max_steps;
+    sp.t0 = pb->t0;
+    sp.t1 = pb->t1;
+    sp.dt0 = pb->dt0;
+    sp.atol = pb->abs_tol;
+    sp.rtol = pb->rel_tol;
+    sp.kconst = pb->system_constant;
+    sp.max_attempts = pb->max_attempts;
+    sp.adaptive = pb->adaptive;
+    va::FwdArgs a{};
+    a.sp = sp;
+    a.y = yT;
+    a.p = p;
+    a.traj = traj;
+    a.n_acc = nacc;
+    a.n_rej = nrej;
+    a.status = stat;
+    a.counter = counters + 0;
+    a.max_steps_seen = counters + 2;
+    VA_HIP(ks.fwd(a, blocks_for(lanes_f, per_block), block, s));
+    va::RevArgs r{};
+    r.sp = sp;
+    r.order = nullptr;
+    r.p = p;
+    r.traj = traj;
+    r.n_acc = nacc;
+    r.status = stat;
+    r.cost = cost;
+    r.cost_w = cost_weights;
+    r.yT = yT;
+    r.J = J;
+    r.lambda = dJdy0;
+    r.mu = dJdp;
+    r.nout = 1;
+    r.counter = counters + 1;
+    VA_HIP(ks.rev(r, blocks_for(lanes_r, per_block), block, s));
+    if (n_accepted || n_rejected || lane_status) {
+        hipLaunchKernelGGL(k_copy_counts, dim3(blocks_for(B, 256)), dim3(256), 0, s, n_accepted,
+                           n_rejected, lane_status, nacc, nrej, stat, B);
+        VA_HIP(hipGetLastError());
+    }
     return VA_OK;
 }
 
