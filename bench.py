@@ -12,6 +12,7 @@ full (t, y, stage K) trajectory store) and the CPU oracle on the host cores.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -86,13 +87,36 @@ def kernel_times(solver, y0, p, reps, torch):
     return float(np.mean(tf)), float(np.mean(tr)), n_acc.cpu().numpy(), n_rej.cpu().numpy()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Hardware threads this process may run on (hardware_concurrency of its
+    affinity mask) and the threads the baseline uses: all of them up to 16, the
+    CPU share a one-GPU job gets on the GPU host."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return avail, int(os.environ.get("VA_CPU_THREADS", min(avail, 16)))
+
+
 def cpu_baseline(w, inputs, max_steps, target_s=10.0, max_reps=20):
     """The CPU oracle timed on the host cores, on the same inputs and step cap as the
     GPU solve: a prefix of the batch sized to about target_s of work or, when the
     whole batch takes less, the whole batch repeated (at most max_reps times) until
-    about target_s.  The rate counts instances that end OK, as the GPU's value."""
+    about target_s.  The rate counts instances that end OK, as the GPU's value.  A
+    one-thread rate on a prefix of about 2 s is reported beside it."""
     import oracle as O
-    nthreads = int(os.environ.get("VA_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    avail, nthreads = host_threads()
     B = inputs["y0"].shape[1]
     pb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
                    abs_tol=w.tol, rel_tol=w.tol, max_steps=max_steps)
@@ -110,10 +134,19 @@ def cpu_baseline(w, inputs, max_steps, target_s=10.0, max_reps=20):
         total += time.perf_counter() - t
         n_ok += int((o["status"] == 0).sum())
         reps += 1
+    # one thread: a prefix sized to about 2 s
+    k1 = int(min(B, max(64, probe * 2.0 / max(dt * nthreads, 1e-6))))
+    t = time.perf_counter()
+    o1 = O.solve_batch(pb, inputs["y0"][:, :k1].copy(), inputs["p"][:, :k1].copy(), w.cost,
+                       nthreads=1)
+    t1 = time.perf_counter() - t
     return {"value": n_ok / total, "unit": "gradients/s", "cores": nthreads, "kind": "port",
             "sample": f"first {k} of {B} instances x {reps} runs (same seeded inputs, step cap "
                       f"{max_steps}; instances that end OK per second), {total:.1f} s wall, "
-                      f"{nthreads} threads"}
+                      f"{nthreads} threads",
+            "host_threads": avail, "host_cpu": cpu_model(),
+            "single_thread": {"value": int((o1["status"] == 0).sum()) / t1,
+                              "sample": f"first {k1} instances, {t1:.1f} s, 1 thread"}}
 
 
 def bench_observed(a, torch, va, I):
@@ -186,7 +219,7 @@ def bench_observed(a, torch, va, I):
         "config": {"workload": "vdp_obs", "batch": B, "tableau": "dopri5", "tol": w.tol,
                    "t_span": [w.t0, w.t1], "n_obs": K_obs, "mu_range": [0.1, 100.0],
                    "trajectory_budget_bytes": budget, "instances_ok": n_ok},
-        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU", "kernel": "solve",
+        "roofline": {"bound": "fp64-valu", "compute": "fp64 vector ALU", "kernel": "solve",
                      "achieved": fl / sec / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": fl / sec / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
                      "flop_per_solve": fl, "note": "forward counted twice (recompute)"},
@@ -315,7 +348,11 @@ def main(argv=None) -> int:
     y0 = torch.from_numpy(inp["y0"]).cuda()
     p = torch.from_numpy(inp["p"]).cuda()
 
-    pb = va.Problem.from_workload(w, max_steps=MAX_STEPS)
+    # launch shape: the project's autotuner (va_autotune) on the whole batch
+    from vectorizedadjoint_amd.autotune import autotune
+    log("[bench] autotuning the launch shape")
+    pb = autotune(va.Problem.from_workload(w, max_steps=MAX_STEPS), y0, p, w.cost, sample=B,
+                  reps=3, use_cache=False)
     log("[bench] persistent form")
     with va.Solver(pb) as s:
         sec, out, solve_ms = time_solver(s, y0, p, w.cost, a.steps, a.warmup, torch)
@@ -327,8 +364,7 @@ def main(argv=None) -> int:
     # (cold), and a parameter set that drifts at every step by a relative 1e-6
     # (an optimizer loop), where the previous solve's counts are a prediction.
     log("[bench] schedule variants: index order; drifting parameters")
-    with va.Solver(va.Problem.from_workload(w, max_steps=MAX_STEPS,
-                                            flags=va.FLAG_COLD_ORDER)) as s:
+    with va.Solver(dataclasses.replace(pb, flags=va.FLAG_COLD_ORDER)) as s:
         sec_cold, out_c, _ = time_solver(s, y0, p, w.cost, a.steps, a.warmup, torch)
         n_ok_cold = int((out_c["status"] == 0).sum().item())
     gen = torch.Generator(device="cuda").manual_seed(12345)
@@ -357,7 +393,9 @@ def main(argv=None) -> int:
         with va.Solver(va.Problem.from_workload(w, max_steps=MAX_STEPS, mode=va.MODE_PLAIN)) as s2:
             psec, _, _ = time_solver(s2, y0, p, w.cost, max(1, a.steps // 4), 1, torch)
         plain = {"value": n_ok / psec, "ms_per_step": psec * 1e3,
-                 "form": "one launch per accepted step, no refill, full (t, y, K) store"}
+                 "form": "one launch per accepted step, no refill, every (t, h, y) row "
+                         "stored; one reverse launch per step recomputing its stages as the "
+                         "reference does"}
 
     cpu = None
     if not a.no_cpu:
@@ -378,9 +416,16 @@ def main(argv=None) -> int:
         "config": {"workload": "glv10", "system": "generalized Lotka-Volterra", "n_state": w.n,
                    "n_param": w.np, "batch": B, "tableau": "cash_karp54", "tol": w.tol,
                    "t_span": [w.t0, w.t1], "cost": "sum_i x_i(T)",
-                   "instances_ok": n_ok, "max_steps": MAX_STEPS},
-        "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (MI355X FP64 vector = "
-                     "matrix peak)", "kernel": "solve",
+                   "instances_ok": n_ok, "max_steps": MAX_STEPS,
+                   "launch": {"block_size": pb.block_size,
+                              "instances_per_wave": pb.instances_per_wave,
+                              "tuned_by": "va_autotune on the whole batch (3 timed solves per "
+                                          "configuration)"},
+                   "order": "repeated solves on one handle: each forward takes instances "
+                            "longest-first by the previous solve's attempts (a handle's "
+                            "first solve runs in index order: schedule.cold_index_order)"},
+        "roofline": {"bound": "fp64-valu", "compute": "fp64 vector ALU (its peak equals the "
+                     "FP64 matrix peak on MI355X)", "kernel": "solve",
                      "kernels": "k_glv_rows_long (one wave per long instance) beside "
                                 "k_glv_rows_fwd then k_glv_rows_rev",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

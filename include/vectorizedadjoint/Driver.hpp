@@ -246,10 +246,20 @@ class Driver
             pb.rel_tol = rel_tol;
         }
         pb.max_steps = cap;
-        if (m_handle && same_problem(pb, m_pb)) return;
+        if (m_handle && same_problem(pb, m_pb) && sys.kernels == m_fwd_system.kernels) return;
         if (m_handle) va_destroy(m_handle);
         m_handle = nullptr;
-        detail_driver::check(va_create(&pb, &m_handle), "va_create");
+        if (sys.id == VA_SYSTEM_USER) {
+            const va_kernel_table *k = sys.kernels ? sys.kernels(tableau, adaptive ? 1 : 0)
+                                                   : nullptr;
+            if (!k)
+                throw std::invalid_argument("runge_kutta: the bound device system has no "
+                                            "kernels for this stepper");
+            detail_driver::check(va_create_with_kernels(&pb, k, &m_handle),
+                                 "va_create_with_kernels");
+        } else {
+            detail_driver::check(va_create(&pb, &m_handle), "va_create");
+        }
         m_pb = pb;
         m_fwd_system = sys;
         m_fwd_tableau = tableau;

@@ -50,8 +50,9 @@ typedef enum va_system {
                                           k = va_problem.system_constant                      */
     VA_SYSTEM_LOTKA_VOLTERRA = 4,      /* examples/GeneralizedLotkaVolterra/main.cpp:105-119;
                                           p = (r[N], A[N][N] row-major), P = N + N*N          */
-    VA_SYSTEM_LOTKA_VOLTERRA_SHARED = 5 /* same system with A shared by the batch: p = r[N],
+    VA_SYSTEM_LOTKA_VOLTERRA_SHARED = 5, /* same system with A shared by the batch: p = r[N],
                                           P = N; A passed to va_solve_shared; N = 128     */
+    VA_SYSTEM_USER = 100               /* a system of the caller's: va_create_with_kernels  */
 } va_system;
 
 /* Tableaus: ButcherTable.hpp:39-251 (Euler, RK4, Cash-Karp 5(4), Fehlberg 7(8))
@@ -101,9 +102,13 @@ typedef struct va_problem {
     int32_t max_attempts;      /* tries per step before VA_LANE_MAX_FAIL (odeint: 500) */
     int32_t max_steps;         /* accepted-step capacity of each stored trajectory    */
     int32_t mode;              /* va_mode                                             */
-    int32_t block_size;        /* threads per block (64..256, multiple of 64), 0 = tuned */
+    int32_t block_size;        /* threads per block (64..256, multiple of 64); 0 = the
+                                  va_autotune choice for this problem shape when this
+                                  process tuned one, else 256                          */
     int32_t instances_per_wave;  /* 64: one lane per instance; 64/N: one group of N
-                                    lanes per instance (Lotka-Volterra); 0 = tuned    */
+                                    lanes per instance (Lotka-Volterra); 0 = the
+                                    va_autotune choice when tuned, else the group
+                                    layout where the system has one                  */
     int32_t checkpoint_slots;  /* observed solves: 0 = store every step while the store
                                   fits trajectory_budget_bytes, else checkpoint; > 0 =
                                   checkpoint with this many slots per instance (even) */
@@ -126,6 +131,16 @@ typedef struct va_problem {
  * last time start first; results do not depend on the order.  With this flag every
  * forward takes instances in index order. */
 #define VA_FLAG_COLD_ORDER 2
+/* va_problem.flags: an observed (or checkpointed) problem keeps two levels of
+ * checkpoints even where one fits trajectory_budget_bytes.  Without the flag two
+ * levels are used only when one does not fit: the forward keeps level-1 checkpoints
+ * (online, doubling their spacing as an instance runs long); the reverse replays a
+ * level-1 segment once to keep about sqrt(segment) level-2 checkpoints per lane,
+ * then recomputes and reverses the rows sub-segment by sub-segment.  Memory per
+ * instance: slots checkpoints; per lane in flight: slots2 checkpoints and
+ * ceil(stride_max / slots2) rows instead of stride_max rows; work: two extra forward
+ * passes instead of one.  Results are the same bit for bit. */
+#define VA_FLAG_CHECKPOINT_2LEVEL 4
 
 typedef struct va_handle va_handle;
 
@@ -137,6 +152,21 @@ void va_problem_init(va_problem *pb);
  * device kernels for (system, N, tableau) and allocates the trajectory store
  * (B * (max_steps+1) rows).  */
 va_status va_create(const va_problem *pb, va_handle **out);
+
+/* Kernels compiled for a system of the caller's (opaque).  Made in the caller's own
+ * hipcc translation unit by VA_DEVICE_SYSTEM (include/vectorizedadjoint/device_system.hpp)
+ * from a device functor with the right-hand side and its vector-Jacobian products. */
+typedef struct va_kernel_table va_kernel_table;
+
+/* recordDriverRHSFunction of an arbitrary functor (lib/include/Driver.hpp:95-100,
+ * lib/include/AadData.hpp:124-171 -- the reference tapes any templated functor with
+ * AADC): va_create for pb->system = VA_SYSTEM_USER with the kernels of the caller's
+ * system.  pb->n_state and pb->n_param must be the functor's N and P; one lane per
+ * instance; every mode and entry point of a one-lane built-in system (va_forward,
+ * va_adjoint, va_sensitivity_matrix, va_solve, va_solve_observed, the plain form) is
+ * available.  A table built against other headers is rejected. */
+va_status va_create_with_kernels(const va_problem *pb, const va_kernel_table *kernels,
+                                 va_handle **out);
 
 /* delete_driver_handle (lib/include/Driver.hpp:81-85). */
 void va_destroy(va_handle *h);
@@ -210,6 +240,24 @@ va_status va_solve_shared(va_handle *h, const double *y0, const double *p, const
  * Synchronises the stream. */
 va_status va_trajectory(va_handle *h, int64_t instance, double *rows, int64_t max_rows,
                         int64_t *n_rows, void *stream);
+
+/* Launch-shape autotuner (no reference counterpart: the reference has no launch
+ * shape).  Times the launch configurations of pb -- threads per block (64, 128, 256),
+ * the instance layouts the system has (one lane per instance, one group of N lanes per
+ * instance) and, for observed problems (pb->n_obs > 0, obs_t / obs_y given), the
+ * checkpoint slots (0, 64, 256) -- on the first min(sample, B) instances of the
+ * caller's own y0 [N][B] and p [P][B], each as a warm-up solve and `reps` timed solves,
+ * and writes the fastest into pb->block_size, pb->instances_per_wave and
+ * pb->checkpoint_slots.  Every configuration computes the same results.  The choice is
+ * also kept for this process per device and problem shape (system, N, tableau,
+ * adaptive, n_obs, batch rounded up to a power of two), and va_create applies it to
+ * problems of that shape whose block_size / instances_per_wave are 0.  kernels: the
+ * table of a VA_SYSTEM_USER problem, else NULL.  Allocates and synchronises the
+ * stream (a set-up call). */
+va_status va_autotune(va_problem *pb, const va_kernel_table *kernels, const double *y0,
+                      const double *p, int32_t cost, const double *cost_weights,
+                      const double *obs_t, const double *obs_y, int64_t sample, int32_t reps,
+                      void *stream);
 
 /* Scratch held by va_solve_noalloc on each device it runs on (until the process ends). */
 #define VA_NOALLOC_SCRATCH_BYTES (32ll << 20)

@@ -20,6 +20,7 @@ SYS_OSCILLATOR = 2
 SYS_OSCILLATOR_FIXED_K = 3
 SYS_LOTKA_VOLTERRA = 4
 SYS_LOTKA_VOLTERRA_SHARED = 5
+SYS_USER = 100
 
 TAB_EULER = 1
 TAB_RK4 = 2
@@ -125,6 +126,8 @@ def lib():
                                              C.c_int64, dp, dp, dp, C.c_int, dp, dp, dp, dp, dp,
                                              dp, ip, ip, ip, C.POINTER(C.c_uint64), C.c_int]
         L.orc_solve_batch_digest.restype = C.c_int
+        L.orc_set_user_system.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_set_user_system.restype = None
         L.orc_step_digest.argtypes = [C.POINTER(Trajectory)]
         L.orc_step_digest.restype = C.c_uint64
         L.orc_forward_obs.argtypes = [C.POINTER(Problem), C.POINTER(Obs), dp, dp,
@@ -225,6 +228,12 @@ def vjp(system, x, p, w, t=0.0, sys_const=0.0):
 def root_(x, q):
     """x^(-1/q) as the step controller computes it (q = 2..8)."""
     return lib().orc_root(float(x), int(q))
+
+
+def set_user_system(rhs_fn, vjp_fn):
+    """Use the C functions rhs_fn(x, p, f) and vjp_fn(x, p, w, wJx, wJp) (ctypes
+    function pointers, e.g. attributes of a loaded shared library) as SYS_USER."""
+    lib().orc_set_user_system(C.cast(rhs_fn, C.c_void_p), C.cast(vjp_fn, C.c_void_p))
 
 
 def rk_step(pb, y, p, t, h):

@@ -301,10 +301,24 @@ static double glv_growth(int n, const double *Ai, double r, const double *x, int
     return r + sum;
 }
 
+static orc_user_rhs g_user_rhs = NULL;
+static orc_user_vjp g_user_vjp = NULL;
+
+void orc_set_user_system(orc_user_rhs rhs, orc_user_vjp vjp)
+{
+    g_user_rhs = rhs;
+    g_user_vjp = vjp;
+}
+
 static void rhs_v(int system, int n, const double *x, const double *p, double t, double K,
                   double *f, int fused)
 {
     (void)t;
+    if (system == ORC_SYS_USER) {
+        if (g_user_rhs) g_user_rhs(x, p, f);
+        else for (int i = 0; i < n; ++i) f[i] = NAN;
+        return;
+    }
     switch (system) {
     case ORC_SYS_VAN_DER_POL: { /* examples/VanDerPol/main.cpp:41-42 */
         double x0 = x[0], x1 = x[1];
@@ -339,6 +353,11 @@ void orc_rhs(int system, int n, const double *x, const double *p, double t, doub
 static void vjp_v(int system, int n, int np, const double *x, const double *p, double t,
                   double K, const double *w, double *wJx, double *wJp, int fused)
 {
+    if (system == ORC_SYS_USER) {
+        if (g_user_vjp) g_user_vjp(x, p, w, wJx, wJp);
+        else for (int i = 0; i < n; ++i) wJx[i] = NAN;
+        return;
+    }
     (void)t;
     (void)np;
     switch (system) {

@@ -72,8 +72,19 @@ enum {
     ORC_SYS_OSCILLATOR = 2,            /* x0' = x1, x1' = -k x0 - c x1; p = (k, c); P = 2      */
     ORC_SYS_OSCILLATOR_FIXED_K = 3,    /* x0' = x1, x1' = -K x0 - mu x1; p = (mu), K constant  */
     ORC_SYS_LOTKA_VOLTERRA = 4,        /* x_i' = x_i (p_i + sum_j p[N(i+1)+j] x_j); P = N+N^2 */
-    ORC_SYS_LOTKA_VOLTERRA_SHARED = 5  /* same right-hand side, A shared by the batch          */
+    ORC_SYS_LOTKA_VOLTERRA_SHARED = 5, /* same right-hand side, A shared by the batch          */
+    ORC_SYS_USER = 100                 /* the callbacks of orc_set_user_system               */
 };
+
+/* A system of the caller's (the device's VA_SYSTEM_USER): f = f(x, p) and
+ * wJx = w^T df/dx, wJp = w^T df/dp (overwritten), the signatures of the device
+ * functor's rhs and vjp. */
+typedef void (*orc_user_rhs)(const double *x, const double *p, double *f);
+typedef void (*orc_user_vjp)(const double *x, const double *p, const double *w, double *wJx,
+                             double *wJp);
+
+/* Sets the process-wide ORC_SYS_USER system (before solving; not thread-safe). */
+void orc_set_user_system(orc_user_rhs rhs, orc_user_vjp vjp);
 
 enum {
     ORC_TAB_EULER = 1,

@@ -10,6 +10,11 @@
 //           controlled, x0 = 0.1, parameters from alphasfile_cpp.csv, t in
 //           [0, 10], dt0 = 1e-3, Nout = N identity seeds; plus the sensitivity
 //           matrix.
+//   user -- a system the library does not know (the Brusselator), its host functor
+//           bound with VA_BIND_DEVICE_SYSTEM to the device functor compiled in
+//           tests/cpp/va_user_system.hip (recordDriverRHSFunction of an arbitrary
+//           functor, Driver.hpp:95-100): DOPRI5 controlled, a = 1, b = 3,
+//           x0 = (1.2, 3.1), t in [0, 10], Nout = N identity seeds.
 //   pre  -- the precondition messages of adjointSolve (backpropagation.hpp:23-42).
 //   fn   -- the batched AdjointFunction (forward / backward over 64 instances).
 //
@@ -64,6 +69,23 @@ struct LotkaVolterra {
         }
     }
 };
+
+// A system of the caller's: the host functor (used by Driver::Rhs) and, bound to
+// it, the device functor's kernels from libva_user_system.so.
+struct Brusselator {
+    template <typename T>
+    void operator()(const std::vector<T> &x, std::vector<T> &dxdt, const std::vector<T> &p, T) const
+    {
+        dxdt[0] = p[0] + x[0] * x[0] * x[1] - (p[1] + 1.0) * x[0];
+        dxdt[1] = p[1] * x[0] - x[0] * x[0] * x[1];
+    }
+};
+
+}  // namespace
+
+VA_BIND_DEVICE_SYSTEM(Brusselator, va_brusselator_kernels)
+
+namespace {
 
 // Counts observer calls and keeps the first and last (t, x).
 struct Recorder {
@@ -241,7 +263,7 @@ int batched()
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::cerr << "usage: " << argv[0] << " vdp|ho|glv|pre|fn [tol] [N] [alphas.csv]\n";
+        std::cerr << "usage: " << argv[0] << " vdp|ho|glv|user|pre|fn [tol] [N] [alphas.csv]\n";
         return 1;
     }
     const std::string which = argv[1];
@@ -266,6 +288,10 @@ int main(int argc, char **argv)
             typedef odeint::runge_kutta_cash_karp54<state> stepper_type;
             run("glv", odeint::make_controlled<stepper_type>(tol, tol), stepper_type(),
                 LotkaVolterra(), state(N, 0.1), alphas, 0.0, 10.0, 1e-3, 1, true);
+        } else if (which == "user") {
+            typedef odeint::runge_kutta_dopri5<state> stepper_type;
+            run("user", odeint::make_controlled<stepper_type>(tol, tol), stepper_type(),
+                Brusselator(), state{1.2, 3.1}, state{1.0, 3.0}, 0.0, 10.0, 1e-3, 1, true);
         } else if (which == "pre") {
             return preconditions();
         } else if (which == "fn") {

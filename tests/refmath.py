@@ -16,6 +16,7 @@ import numpy as np
 
 SYS_VAN_DER_POL, SYS_OSCILLATOR, SYS_OSCILLATOR_FIXED_K, SYS_LOTKA_VOLTERRA = 1, 2, 3, 4
 SYS_LOTKA_VOLTERRA_SHARED = 5
+SYS_USER = 100  # in the tests: the Brusselator of tests/cpp/va_user_system.hip
 
 
 def rooted_trees(max_order: int):
@@ -67,6 +68,9 @@ def rhs(system, x, p, K=0.0):
         return np.stack([x[1], -p[0] * x[0] - p[1] * x[1]])
     if system == SYS_OSCILLATOR_FIXED_K:
         return np.stack([x[1], -K * x[0] - p[0] * x[1]])
+    if system == SYS_USER:  # Brusselator: x0' = a + x0^2 x1 - (b+1) x0, x1' = b x0 - x0^2 x1
+        a, b = p[0], p[1]
+        return np.stack([a + x[0] ** 2 * x[1] - (b + 1.0) * x[0], b * x[0] - x[0] ** 2 * x[1]])
     if system in (SYS_LOTKA_VOLTERRA, SYS_LOTKA_VOLTERRA_SHARED):
         N = x.shape[0]
         r = p[:N]

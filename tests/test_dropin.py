@@ -27,6 +27,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 TOL = 1e-10
 
 IDENTIFY_SRC = r"""
+#include <cmath>
 #include <cstdio>
 #include <vector>
 #include "vectorizedadjoint/lib.hpp"
@@ -46,13 +47,23 @@ struct Other { template <class T> void operator()(const std::vector<T>&x, std::v
 struct Bound { template <class T> void operator()(const std::vector<T>&, std::vector<T>&f,
     const std::vector<T>&, T) const { f[0]=0; f[1]=0; } };
 VA_BIND_SYSTEM(Bound, VA_SYSTEM_VAN_DER_POL, 0.0)
+// forced oscillator: the damped oscillator plus a forcing term in t
+struct Forced { template <class T> void operator()(const std::vector<T>&r, std::vector<T>&f,
+    const std::vector<T>&p, T t) const { f[0]=r[1]; f[1]=-p[0]*r[0]-p[1]*r[1]+std::sin(t); } };
+// piecewise: Van der Pol below x0 = 2.5, something else above
+struct Piecewise { template <class T> void operator()(const std::vector<T>&x, std::vector<T>&f,
+    const std::vector<T>&p, T) const { f[0]=x[1];
+      f[1] = x[0] < 2.5 ? p[0]*((1.0-x[0]*x[0])*x[1]-x[0]) : -x[0]; } };
 int main() {
-  Vdp a; Ho b; Osc c; Glv d; Other e; Bound g;
+  Vdp a; Ho b; Osc c; Glv d; Other e; Bound g; Forced fo; Piecewise pw;
   auto s1 = identify_system(a, 2, 1), s2 = identify_system(b, 2, 1), s3 = identify_system(c, 2, 2);
   auto s4 = identify_system(d, 10, 110), s6 = identify_system(g, 2, 1);
-  int other = 0;
+  int other = 0, forced = 0, piecewise = 0;
   try { identify_system(e, 2, 1); } catch (std::invalid_argument &) { other = 1; }
-  std::printf("%d %d %.17g %d %d %d %d\n", s1.id, s2.id, s2.constant, s3.id, s4.id, s6.id, other);
+  try { identify_system(fo, 2, 2); } catch (std::invalid_argument &) { forced = 1; }
+  try { identify_system(pw, 2, 1); } catch (std::invalid_argument &) { piecewise = 1; }
+  std::printf("%d %d %.17g %d %d %d %d %d %d\n", s1.id, s2.id, s2.constant, s3.id, s4.id, s6.id,
+              other, forced, piecewise);
   return 0;
 }
 """
@@ -64,8 +75,10 @@ def _compile(src, out, tmp_path):
     cmd = ["g++", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
            "-I", "/opt/rocm/include", str(f), "-o", str(out),
            "-L", os.path.join(ROOT, "vectorizedadjoint_amd"), "-lvectorizedadjoint_amd",
+           "-L", os.path.join(ROOT, "tests", "cpp"), "-lva_user_system",
            "-L", "/opt/rocm/lib", "-lamdhip64",
-           f"-Wl,-rpath,{os.path.join(ROOT, 'vectorizedadjoint_amd')}", "-Wl,-rpath,/opt/rocm/lib"]
+           f"-Wl,-rpath,{os.path.join(ROOT, 'vectorizedadjoint_amd')}",
+           f"-Wl,-rpath,{os.path.join(ROOT, 'tests', 'cpp')}", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
 
@@ -85,6 +98,8 @@ def test_headers_compile_and_identify_systems(tmp_path):
     assert int(ids[4]) == K.SYS_LOTKA_VOLTERRA
     assert int(ids[5]) == K.SYS_VAN_DER_POL      # explicit VA_BIND_SYSTEM
     assert int(ids[6]) == 1                      # unknown functor -> std::invalid_argument
+    assert int(ids[7]) == 1                      # time-dependent (forced) functor refused
+    assert int(ids[8]) == 1                      # piecewise functor refused
 
 
 def test_example_source_builds_against_headers(tmp_path):
@@ -163,6 +178,23 @@ def test_example_lotka_volterra_cash_karp(n):
     res = json.loads(_run_example("glv", "1e-5", str(n), csv))
     pb = O.problem(K.SYS_LOTKA_VOLTERRA, n, n + n * n, K.TAB_CASH_KARP54, t0=0.0, t1=10.0,
                    dt0=1e-3, abs_tol=1e-5, rel_tol=1e-5)
+    _check_against_oracle(res, pb, True)
+
+
+@pytest.mark.gpu
+def test_example_user_system_brusselator():
+    """A functor the library does not know, through the unchanged reference API:
+    recordDriverRHSFunction / runge_kutta bind it (VA_BIND_DEVICE_SYSTEM) to the
+    device functor compiled outside the library; the oracle runs the same functor's
+    host build as its callbacks."""
+    import ctypes
+    res = json.loads(_run_example("user", "1e-8"))
+    # the callbacks come from the functor's host build, a library without device
+    # code: the device library is the example's
+    lib = ctypes.CDLL(os.path.join(ROOT, "tests", "cpp", "libva_brusselator_host.so"))
+    O.set_user_system(lib.brusselator_rhs, lib.brusselator_vjp)
+    pb = O.problem(O.SYS_USER, 2, 2, K.TAB_DOPRI5, t0=0.0, t1=10.0, dt0=1e-3,
+                   abs_tol=1e-8, rel_tol=1e-8)
     _check_against_oracle(res, pb, True)
 
 

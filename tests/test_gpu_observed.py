@@ -75,6 +75,35 @@ def test_checkpointed_reverse_is_bit_identical_to_full_store():
             assert np.array_equal(full[k], ck[k]), (slots, k)
 
 
+def test_two_level_checkpoints_are_bit_identical_to_full_store():
+    """Two levels of checkpoints (VA_FLAG_CHECKPOINT_2LEVEL): the reverse replays
+    each long segment once to keep level-2 checkpoints, then recomputes its rows
+    sub-segment by sub-segment; every output equals the full store's."""
+    B = 512
+    torch, O, va, w, inp, t_obs, y_obs, pb = _setup(B, t1=12.0, K=30)
+    full = _gpu(torch, va, w, inp, t_obs, y_obs, 12.0)
+    assert full["n_acc"].max() > 1000
+    for slots in (2, 6, 0):
+        kw = {"flags": va.FLAG_CHECKPOINT_2LEVEL, "checkpoint_slots": slots}
+        if slots == 0:
+            kw["trajectory_budget_bytes"] = 64 << 20
+        ck = _gpu(torch, va, w, inp, t_obs, y_obs, 12.0, **kw)
+        for k in ("J", "yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status"):
+            assert np.array_equal(full[k], ck[k]), (slots, k)
+
+
+def test_budget_below_one_level_uses_two_levels():
+    """A budget under the smallest one-level checkpoint store (about 4.2 MB here:
+    slots x 40 B per instance plus stride_max rows per lane) but above the
+    two-level one (about 1.4 MB) is accepted and gives the full store's bits."""
+    B = 512
+    torch, O, va, w, inp, t_obs, y_obs, pb = _setup(B, t1=12.0, K=30)
+    full = _gpu(torch, va, w, inp, t_obs, y_obs, 12.0)
+    ck = _gpu(torch, va, w, inp, t_obs, y_obs, 12.0, trajectory_budget_bytes=2 << 20)
+    for k in ("J", "yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status"):
+        assert np.array_equal(full[k], ck[k]), k
+
+
 def test_budget_selects_checkpoints_and_reports_state_errors():
     torch, O, va, w, inp, t_obs, y_obs, pb = _setup(256, t1=8.0, K=10)
     B = 256

@@ -219,3 +219,23 @@ def test_fixed_step_euler_matches_oracle(name, ipw):
     assert (o["status"] == 0).all() and (o["n_acc"] > 400).all()
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096, instances_per_wave=ipw)
     _compare(g, o)
+
+
+@pytest.mark.parametrize("n,tableau", [(7, "cash_karp54"), (16, "cash_karp54"),
+                                       (32, "cash_karp54"), (5, "dopri5"), (13, "fehlberg78")])
+def test_glv_any_n_rows_match_oracle(n, tableau):
+    """Generalized Lotka-Volterra at other N (the reference's driver reads N from the
+    command line, examples/GeneralizedLotkaVolterra/main.cpp:125-143): one group of
+    N lanes per instance, same distributions as the N=10 workload over t in [0, 10]."""
+    import dataclasses
+    from vectorizedadjoint_amd import (inputs as I, MODE_PERSISTENT, TAB_CASH_KARP54, TAB_DOPRI5,
+                                       TAB_FEHLBERG78)
+    tid = {"cash_karp54": TAB_CASH_KARP54, "dopri5": TAB_DOPRI5,
+           "fehlberg78": TAB_FEHLBERG78}[tableau]
+    w = dataclasses.replace(I.WORKLOADS["glv10"], name=f"glv{n}", n=n, np=n + n * n, tableau=tid,
+                            t1=10.0)
+    inp = I.make_inputs(w, 900)
+    o = run_oracle(w, inp, max_steps=1024)
+    assert (o["status"] == 0).sum() > 450
+    g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=1024)
+    _compare(g, o)

@@ -31,6 +31,7 @@ C_ABI_FUNCTIONS = (
     "va_problem_init", "va_create", "va_destroy", "va_forward", "va_adjoint",
     "va_sensitivity_matrix", "va_solve", "va_solve_observed", "va_solve_shared", "va_trajectory",
     "va_step_digest", "va_max_steps_seen", "va_last_error", "va_solve_noalloc",
+    "va_create_with_kernels", "va_autotune",
 )
 
 
@@ -73,6 +74,9 @@ def lib() -> C.CDLL:
         L.va_problem_init.argtypes = [C.POINTER(VaProblem)]
         L.va_problem_init.restype = None
         L.va_create.argtypes = [C.POINTER(VaProblem), C.POINTER(vp)]
+        L.va_create_with_kernels.argtypes = [C.POINTER(VaProblem), vp, C.POINTER(vp)]
+        L.va_autotune.argtypes = [C.POINTER(VaProblem), vp, vp, vp, C.c_int32, vp, vp, vp,
+                                  C.c_int64, C.c_int32, vp]
         L.va_destroy.argtypes = [vp]
         L.va_destroy.restype = None
         L.va_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -87,7 +91,7 @@ def lib() -> C.CDLL:
         L.va_trajectory.argtypes = [vp, C.c_int64, vp, C.c_int64, C.POINTER(C.c_int64), vp]
         L.va_last_error.argtypes = []
         L.va_last_error.restype = C.c_char_p
-        for f in ("va_create", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
+        for f in ("va_create", "va_create_with_kernels", "va_autotune", "va_forward", "va_adjoint", "va_sensitivity_matrix", "va_solve",
                   "va_solve_observed", "va_solve_shared", "va_trajectory", "va_max_steps_seen",
                   "va_step_digest", "va_solve_noalloc"):
             getattr(L, f).restype = C.c_int
@@ -173,12 +177,18 @@ class Solver:
     index fastest: y [N, B], p [P, B], lambda [Nout, N, B], mu [Nout, P, B].
     """
 
-    def __init__(self, problem: Problem):
+    def __init__(self, problem: Problem, kernels=None):
+        """kernels: for problem.system == SYS_USER, the va_kernel_table pointer that a
+        caller's VA_DEVICE_SYSTEM function returned (an int or ctypes pointer)."""
         self.problem = problem
         self._h = C.c_void_p()
         self._pb = problem.to_c()
         self.generation = 0  # forwards run so far; the reverse uses the last one's trajectory
-        _check(lib().va_create(C.byref(self._pb), C.byref(self._h)))
+        if kernels is None:
+            _check(lib().va_create(C.byref(self._pb), C.byref(self._h)))
+        else:
+            _check(lib().va_create_with_kernels(C.byref(self._pb), C.c_void_p(kernels),
+                                                C.byref(self._h)))
 
     def close(self) -> None:
         if self._h:

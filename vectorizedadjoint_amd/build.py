@@ -23,6 +23,10 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvectorizedadjoint_amd.so")
 EXAMPLES_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "va_examples.cpp")
 EXAMPLES_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "va_examples")
+USER_SYSTEM_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "va_user_system.hip")
+USER_SYSTEM_LIB = os.path.join(os.path.dirname(HERE), "tests", "cpp", "libva_user_system.so")
+USER_HOST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "brusselator_host.cpp")
+USER_HOST_LIB = os.path.join(os.path.dirname(HERE), "tests", "cpp", "libva_brusselator_host.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("VA_OFFLOAD_ARCH", "gfx950")
 
@@ -94,16 +98,45 @@ def build(incremental: bool = False, jobs: int | None = None, verbose: bool = Tr
     return LIB
 
 
+def build_user_system(verbose: bool = True) -> str:
+    """Compile the test suite's own device system (tests/cpp/va_user_system.hip, the
+    Brusselator) the way a caller compiles one: hipcc, the library's kernel
+    templates included through vectorizedadjoint/device_system.hpp.  The same
+    functor's host build (tests/cpp/brusselator_host.cpp, host compiler only:
+    a library without device code) gives the oracle its callbacks."""
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", CSRC, "-x", "hip", "-shared", USER_SYSTEM_SRC,
+                                     "-o", USER_SYSTEM_LIB]
+    t = time.time()
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        msg = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+        raise RuntimeError(f"user system build failed:\n{msg}")
+    cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
+    cmd = [cxx, "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", USER_HOST_SRC,
+           "-o", USER_HOST_LIB]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"user system host build failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {USER_SYSTEM_LIB} and {USER_HOST_LIB} in {time.time() - t:.1f}s",
+              flush=True)
+    return USER_SYSTEM_LIB
+
+
 def build_examples(verbose: bool = True) -> str:
     """Compile the drop-in C++ example drivers (tests/cpp/va_examples.cpp) with the
     host compiler against the C++ headers and the built library: the headers are
-    plain C++17 over the C ABI and the HIP runtime API, no device code."""
+    plain C++17 over the C ABI and the HIP runtime API, no device code.  The
+    'user' scenario's device system comes from tests/cpp/libva_user_system.so
+    (build_user_system first)."""
     cxx = os.environ.get("CXX", shutil.which("g++") or "g++")
     libdir = os.path.relpath(HERE, os.path.dirname(EXAMPLES_BIN))
     cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE,
            "-I", os.path.join(ROCM, "include"), EXAMPLES_SRC, "-o", EXAMPLES_BIN,
-           "-L", HERE, "-lvectorizedadjoint_amd", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
-           f"-Wl,-rpath,$ORIGIN/{libdir}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+           "-L", HERE, "-lvectorizedadjoint_amd", "-L", os.path.dirname(EXAMPLES_BIN),
+           "-lva_user_system", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+           f"-Wl,-rpath,$ORIGIN/{libdir}", "-Wl,-rpath,$ORIGIN",
+           f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
     t = time.time()
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -119,6 +152,7 @@ def main(argv=None) -> int:
     ap.add_argument("-j", "--jobs", type=int, default=None)
     a = ap.parse_args(argv)
     build(incremental=a.incremental, jobs=a.jobs)
+    build_user_system()
     build_examples()
     return 0
 

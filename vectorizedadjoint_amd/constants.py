@@ -6,6 +6,7 @@ SYS_OSCILLATOR = 2
 SYS_OSCILLATOR_FIXED_K = 3
 SYS_LOTKA_VOLTERRA = 4
 SYS_LOTKA_VOLTERRA_SHARED = 5
+SYS_USER = 100  # a system of the caller's, with its own kernel table
 
 # va_tableau
 TAB_EULER = 1
@@ -32,6 +33,7 @@ MODE_PLAIN = 1
 # va_problem.flags
 FLAG_GRAPH = 1
 FLAG_COLD_ORDER = 2  # every forward takes instances in index order (no schedule reuse)
+FLAG_CHECKPOINT_2LEVEL = 4  # two levels of checkpoints even where one fits the budget
 
 # va_status
 STATUS_NAMES = {

@@ -12,7 +12,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <vector>
 
@@ -247,7 +249,8 @@ struct va_handle {
     // observed solves
     int64_t lanes_obs_fwd = 0, lanes_obs_rev = 0;
     int32_t obs_ckpt = 0, obs_slots = 0, obs_stride_max = 0;
-    double *obs_ck = nullptr, *obs_seg = nullptr;
+    int32_t obs_levels = 1, obs_slots2 = 0, obs_seg_rows = 0;
+    double *obs_ck = nullptr, *obs_seg = nullptr, *obs_ck2 = nullptr;
     int32_t *obs_stride = nullptr, *obs_row = nullptr;
     // shared-parameter solves
     int shared_grid = 0;
@@ -339,6 +342,7 @@ void release(va_handle *h)
     hipFree(h->order_tmp);
     hipFree(h->obs_ck);
     hipFree(h->obs_seg);
+    hipFree(h->obs_ck2);
     hipFree(h->obs_stride);
     hipFree(h->obs_row);
     hipFree(h->abar_part);
@@ -657,8 +661,10 @@ const char *va_last_error(void) { return g_err; }
 namespace {
 
 // The checks of va_create that need no device state: argument ranges, the kernel
-// set for (system, N, tableau), the parameter count.
-va_status check_problem(const va_problem *pb, va::KernelSet *ks_out)
+// set for (system, N, tableau) -- or the caller's own table for VA_SYSTEM_USER --
+// and the parameter count.
+va_status check_problem(const va_problem *pb, va::KernelSet *ks_out,
+                        const va::KernelSet *given = nullptr)
 {
     if (pb->abi_version != VA_ABI_VERSION)
         return fail(VA_ERR_INVALID, "abi_version %d, library implements %d", pb->abi_version,
@@ -678,7 +684,7 @@ va_status check_problem(const va_problem *pb, va::KernelSet *ks_out)
         return fail(VA_ERR_INVALID,
                     "tableau %d has no embedded error estimate: use adaptive = 0", pb->tableau);
     va::KernelSet ks{};
-    if (pb->flags & ~(VA_FLAG_GRAPH | VA_FLAG_COLD_ORDER))
+    if (pb->flags & ~(VA_FLAG_GRAPH | VA_FLAG_COLD_ORDER | VA_FLAG_CHECKPOINT_2LEVEL))
         return fail(VA_ERR_INVALID, "unknown va_problem.flags bits");
     if (pb->n_obs < 0) return fail(VA_ERR_INVALID, "n_obs must be >= 0");
     if (pb->checkpoint_slots < 0 || pb->checkpoint_slots == 1 || pb->checkpoint_slots % 2 != 0)
@@ -686,12 +692,31 @@ va_status check_problem(const va_problem *pb, va::KernelSet *ks_out)
 
     if (pb->instances_per_wave < 0 || pb->instances_per_wave > 64)
         return fail(VA_ERR_INVALID, "instances_per_wave must be in [0, 64]");
-    if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive,
-                         pb->instances_per_wave, &ks))
+    if (given) {
+        if (pb->system != VA_SYSTEM_USER)
+            return fail(VA_ERR_INVALID, "a kernel table needs system VA_SYSTEM_USER");
+        if (given->table_size != (int32_t)sizeof(va::KernelSet) ||
+            given->table_version != va::VA_KERNEL_TABLE_VERSION || !given->fwd || !given->rev)
+            return fail(VA_ERR_INVALID,
+                        "kernel table of size %d, version %d: this library takes size %d, "
+                        "version %d (rebuild the system's translation unit against its headers)",
+                        given->table_size, given->table_version, (int)sizeof(va::KernelSet),
+                        va::VA_KERNEL_TABLE_VERSION);
+        if (given->n != pb->n_state)
+            return fail(VA_ERR_INVALID, "the kernel table is for N=%d, the problem says %d",
+                        given->n, pb->n_state);
+        if (pb->instances_per_wave != 0 && pb->instances_per_wave != 64)
+            return fail(VA_ERR_INVALID, "user systems run one lane per instance");
+        ks = *given;
+    } else if (pb->system == VA_SYSTEM_USER) {
+        return fail(VA_ERR_INVALID, "VA_SYSTEM_USER needs va_create_with_kernels");
+    } else if (!va::get_kernels(pb->system, pb->n_state, pb->tableau, pb->adaptive,
+                                pb->instances_per_wave, &ks)) {
         return fail(VA_ERR_UNSUPPORTED,
                     "no kernels for system %d, N=%d, tableau %d, adaptive %d, "
                     "instances_per_wave %d",
                     pb->system, pb->n_state, pb->tableau, pb->adaptive, pb->instances_per_wave);
+    }
     if (pb->n_param != ks.np)
         return fail(VA_ERR_INVALID, "system %d with N=%d has %d parameters, problem says %d",
                     pb->system, pb->n_state, ks.np, pb->n_param);
@@ -699,17 +724,52 @@ va_status check_problem(const va_problem *pb, va::KernelSet *ks_out)
     return VA_OK;
 }
 
-}  // namespace
+// Launch shapes picked by va_autotune in this process, per device and problem
+// shape (system, N, tableau, adaptive, observations, batch rounded up to a power
+// of two).  va_create applies them to problems whose block_size /
+// instances_per_wave are 0.
+struct TuneKey {
+    int device, system, n, tableau, adaptive, n_obs;
+    int64_t bucket;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(device, system, n, tableau, adaptive, n_obs, bucket) <
+               std::tie(o.device, o.system, o.n, o.tableau, o.adaptive, o.n_obs, o.bucket);
+    }
+};
+struct TuneChoice {
+    int32_t block, ipw, slots;
+};
+std::mutex g_tune_mtx;
+std::map<TuneKey, TuneChoice> g_tuned;
 
-extern "C" {
+TuneKey tune_key(const va_problem *pb, int device)
+{
+    int64_t bucket = 1;
+    while (bucket < pb->batch) bucket *= 2;
+    return TuneKey{device, pb->system, pb->n_state, pb->tableau, pb->adaptive, pb->n_obs, bucket};
+}
 
-va_status va_create(const va_problem *pb, va_handle **out)
+va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_handle **out)
 {
     g_err[0] = 0;
-    if (!pb || !out) return fail(VA_ERR_INVALID, "null argument");
+    if (!pb_in || !out) return fail(VA_ERR_INVALID, "null argument");
     *out = nullptr;
+    va_problem tuned = *pb_in;
+    if (tuned.block_size == 0 || tuned.instances_per_wave == 0) {
+        int dev = -1;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            std::lock_guard<std::mutex> lock(g_tune_mtx);
+            auto it = g_tuned.find(tune_key(&tuned, dev));
+            if (it != g_tuned.end()) {
+                if (tuned.block_size == 0) tuned.block_size = it->second.block;
+                if (tuned.instances_per_wave == 0) tuned.instances_per_wave = it->second.ipw;
+            }
+        }
+    }
+    const va_problem *pb = &tuned;
     va::KernelSet ks{};
-    if (va_status st = check_problem(pb, &ks)) return st;
+    if (va_status st = check_problem(pb, &ks, given)) return st;
     if (ks.layout == 2 && (pb->mode != VA_MODE_PERSISTENT || pb->n_obs > 0))
         return fail(VA_ERR_UNSUPPORTED,
                     "shared-parameter systems run in the persistent mode without observations");
@@ -724,10 +784,11 @@ va_status va_create(const va_problem *pb, va_handle **out)
     h->pb = *pb;
     h->ks = ks;
     hipDeviceProp_t prop;
-    if (hipGetDevice(&h->device) != hipSuccess ||
-        hipGetDeviceProperties(&prop, h->device) != hipSuccess) {
+    hipError_t de = hipGetDevice(&h->device);
+    if (de == hipSuccess) de = hipGetDeviceProperties(&prop, h->device);
+    if (de != hipSuccess) {
         delete h;
-        return fail(VA_ERR_HIP, "no HIP device");
+        return fail(VA_ERR_HIP, "no HIP device: %s", hipGetErrorString(de));
     }
     h->cu_count = prop.multiProcessorCount;
     h->block = pb->block_size > 0 ? pb->block_size : 256;
@@ -781,23 +842,54 @@ va_status va_create(const va_problem *pb, va_handle **out)
                 return 8.0 * (double)B * slots * (ks.n + 3) +
                        8.0 * (double)rev_threads * sm * (ks.n + 1) + fixed;
             };
-            int32_t slots = pb->checkpoint_slots, sm = 0;
+            // two levels: within a segment of up to stride_max steps, slots2 level-2
+            // checkpoints per lane (about sqrt(stride_max)) and a row buffer of
+            // ceil(stride_max / slots2) rows
+            auto cost2 = [&](int32_t slots, int32_t *stride_max, int32_t *slots2,
+                             int32_t *seg_rows) {
+                int32_t sm = 1;
+                while ((int64_t)sm * slots < (int64_t)pb->max_steps) sm *= 2;
+                int32_t s2 = 2;
+                while ((int64_t)s2 * s2 < sm) s2 *= 2;
+                *stride_max = sm;
+                *slots2 = s2;
+                *seg_rows = (sm + s2 - 1) / s2;
+                return 8.0 * (double)B * slots * (ks.n + 3) +
+                       8.0 * (double)rev_threads *
+                           ((double)s2 * (ks.n + 3) + (double)*seg_rows * (ks.n + 1)) +
+                       fixed;
+            };
+            const bool force2 = (pb->flags & VA_FLAG_CHECKPOINT_2LEVEL) != 0;
+            int32_t slots = pb->checkpoint_slots, sm = 0, s2 = 0, rows2 = 0;
+            int32_t levels = force2 ? 2 : 1;
             if (slots > 0) {
-                cost(slots, &sm);
+                if (levels == 1) cost(slots, &sm);
+                else cost2(slots, &sm, &s2, &rows2);
             } else {
-                // the most slots that fit (shortest segments, smallest recompute buffers)
-                for (slots = 1 << 16; slots >= 2; slots /= 2)
-                    if (cost(slots, &sm) <= budget) break;
-                if (slots < 2) {
+                // the most slots that fit (shortest segments, smallest recompute buffers):
+                // one level if any fits, else two
+                slots = 0;
+                if (levels == 1)
+                    for (int32_t c = 1 << 16; c >= 2 && !slots; c /= 2)
+                        if (cost(c, &sm) <= budget) slots = c;
+                if (!slots) {
+                    levels = 2;
+                    for (int32_t c = 1 << 16; c >= 2 && !slots; c /= 2)
+                        if (cost2(c, &sm, &s2, &rows2) <= budget) slots = c;
+                }
+                if (!slots) {
                     delete h;
                     return fail(VA_ERR_OUT_OF_MEMORY,
                                 "trajectory_budget_bytes %lld cannot hold checkpoints for %lld "
-                                "instances",
+                                "instances (at two levels either)",
                                 (long long)pb->trajectory_budget_bytes, (long long)B);
                 }
             }
             h->obs_slots = slots;
             h->obs_stride_max = sm;
+            h->obs_levels = levels;
+            h->obs_slots2 = levels == 2 ? s2 : 0;
+            h->obs_seg_rows = levels == 2 ? rows2 : sm;
         }
     } else if (over) {
         delete h;
@@ -826,7 +918,10 @@ va_status va_create(const va_problem *pb, va_handle **out)
             alloc((void **)&h->obs_ck, sizeof(double) * (size_t)B * h->obs_slots * (ks.n + 3));
             alloc((void **)&h->obs_stride, sizeof(int32_t) * (size_t)B);
             alloc((void **)&h->obs_seg,
-                  sizeof(double) * (size_t)rev_threads * h->obs_stride_max * (ks.n + 1));
+                  sizeof(double) * (size_t)rev_threads * h->obs_seg_rows * (ks.n + 1));
+            if (h->obs_levels == 2)
+                alloc((void **)&h->obs_ck2,
+                      sizeof(double) * (size_t)rev_threads * h->obs_slots2 * (ks.n + 3));
         }
     }
     if (ks.layout == 2) {
@@ -883,6 +978,22 @@ va_status va_create(const va_problem *pb, va_handle **out)
     }
     *out = h;
     return VA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+va_status va_create(const va_problem *pb, va_handle **out)
+{
+    return create_handle(pb, nullptr, out);
+}
+
+va_status va_create_with_kernels(const va_problem *pb, const va_kernel_table *kernels,
+                                 va_handle **out)
+{
+    if (!kernels) return fail(VA_ERR_INVALID, "null kernel table");
+    return create_handle(pb, reinterpret_cast<const va::KernelSet *>(kernels), out);
 }
 
 void va_destroy(va_handle *h)
@@ -1084,6 +1195,10 @@ static va_status observed_enqueue(va_handle *h, const double *y0, const double *
     os.stride_max = h->obs_stride_max;
     os.stride = h->obs_stride;
     os.seg = h->obs_seg;
+    os.levels = h->obs_levels;
+    os.seg_rows = h->obs_seg_rows;
+    os.slots2 = h->obs_slots2;
+    os.ck2 = h->obs_ck2;
     os.obs_row = h->obs_row;
     va::ObsFwdArgs f;
     f.sp = solve_params(h);
@@ -1291,6 +1406,117 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 extern "C" {
 
+va_status va_autotune(va_problem *pb, const va_kernel_table *kernels, const double *y0,
+                      const double *p, int32_t cost, const double *cost_weights,
+                      const double *obs_t, const double *obs_y, int64_t sample, int32_t reps,
+                      void *stream)
+{
+    g_err[0] = 0;
+    if (!pb || !y0 || !p || sample < 1 || reps < 1)
+        return fail(VA_ERR_INVALID, "va_autotune needs pb, y0, p, sample >= 1 and reps >= 1");
+    if (pb->n_obs > 0 && (!obs_t || !obs_y))
+        return fail(VA_ERR_INVALID, "an observed problem needs obs_t and obs_y");
+    va::KernelSet ks{};
+    {
+        va_problem probe = *pb;
+        if (va_status st = check_problem(&probe, &ks,
+                                         reinterpret_cast<const va::KernelSet *>(kernels)))
+            return st;
+    }
+    if (ks.layout == 2) return fail(VA_ERR_UNSUPPORTED, "shared-parameter systems are not tuned");
+    int dev = -1;
+    VA_HIP(hipGetDevice(&dev));
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t B = pb->batch, Bs = std::min<int64_t>(B, sample);
+    const int n = pb->n_state, np = pb->n_param;
+    // the first Bs instances, compacted: [field][Bs]
+    double *buf = nullptr;
+    const size_t nd = (size_t)Bs * (size_t)(n + np + 1 + n + n + np);
+    VA_HIP(hipMalloc((void **)&buf, sizeof(double) * nd));
+    double *ys = buf, *ps = ys + (size_t)n * Bs, *J = ps + (size_t)np * Bs, *yT = J + Bs,
+           *gy = yT + (size_t)n * Bs, *gp = gy + (size_t)n * Bs;
+    hipError_t e = hipMemcpy2DAsync(ys, sizeof(double) * Bs, y0, sizeof(double) * B,
+                                    sizeof(double) * Bs, n, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(ps, sizeof(double) * Bs, p, sizeof(double) * B, sizeof(double) * Bs,
+                             np, hipMemcpyDeviceToDevice, s);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&ev0);
+    if (e == hipSuccess) e = hipEventCreate(&ev1);
+    auto cleanup = [&]() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        (void)hipFree(buf);
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(VA_ERR_HIP, "va_autotune set-up: %s", hipGetErrorString(e));
+    }
+    // candidates: threads per block; the instance layouts that exist; checkpoint slots
+    std::vector<int32_t> ipws = {0};
+    if (!kernels) {
+        // one lane per instance as well, where it is a different kernel set
+        va::KernelSet k0{}, k64{};
+        const bool has0 = va::get_kernels(pb->system, n, pb->tableau, pb->adaptive, 0, &k0);
+        const bool has64 = va::get_kernels(pb->system, n, pb->tableau, pb->adaptive, 64, &k64);
+        if (has64 && (!has0 || k64.fwd != k0.fwd)) ipws.push_back(64);
+    }
+    std::vector<int32_t> slots = {pb->checkpoint_slots};
+    if (pb->n_obs > 0) slots = {0, 64, 256};
+    TuneChoice best{0, 0, pb->checkpoint_slots};
+    float best_ms = 1e30f;
+    for (int32_t ipw : ipws)
+        for (int32_t block : {64, 128, 256})
+            for (int32_t sl : slots) {
+                va_problem c = *pb;
+                c.batch = Bs;
+                c.block_size = block;
+                c.instances_per_wave = ipw;
+                c.checkpoint_slots = sl;
+                c.lanes = 0;
+                c.flags &= ~VA_FLAG_GRAPH;
+                va_handle *h = nullptr;
+                if (create_handle(&c, reinterpret_cast<const va::KernelSet *>(kernels), &h) !=
+                    VA_OK)
+                    continue;  // e.g. checkpoints that do not fit the budget
+                auto run = [&]() {
+                    return c.n_obs > 0
+                               ? va_solve_observed(h, ys, ps, obs_t, obs_y, J, yT, gy, gp, nullptr,
+                                                   nullptr, nullptr, stream)
+                               : va_solve(h, ys, ps, cost, cost_weights, J, yT, gy, gp, nullptr,
+                                          nullptr, nullptr, stream);
+                };
+                va_status st = run();  // warm-up: also orders the next solve's instances
+                if (st == VA_OK && hipEventRecord(ev0, s) != hipSuccess) st = VA_ERR_HIP;
+                for (int r = 0; r < reps && st == VA_OK; ++r) st = run();
+                float ms = 0.0f;
+                if (st == VA_OK && (hipEventRecord(ev1, s) != hipSuccess ||
+                                    hipEventSynchronize(ev1) != hipSuccess ||
+                                    hipEventElapsedTime(&ms, ev0, ev1) != hipSuccess))
+                    st = VA_ERR_HIP;
+                va_destroy(h);
+                if (st != VA_OK) {
+                    cleanup();
+                    return st;
+                }
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = TuneChoice{block, ipw, sl};
+                }
+            }
+    cleanup();
+    if (best_ms >= 1e30f) return fail(VA_ERR_UNSUPPORTED, "no launch configuration ran");
+    pb->block_size = best.block;
+    pb->instances_per_wave = best.ipw;
+    pb->checkpoint_slots = best.slots;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mtx);
+        g_tuned[tune_key(pb, dev)] = best;
+    }
+    g_err[0] = 0;
+    return VA_OK;
+}
+
 va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double *p, int32_t cost,
                            const double *cost_weights, double *J, double *yT, double *dJdy0,
                            double *dJdp, int32_t *n_accepted, int32_t *n_rejected,
@@ -1335,7 +1561,8 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
                     "trajectory store of %.3g bytes exceeds trajectory_budget_bytes %lld",
                     traj_bytes, (long long)pb->trajectory_budget_bytes);
     int dev = -1;
-    if (hipGetDevice(&dev) != hipSuccess) return fail(VA_ERR_HIP, "no HIP device");
+    if (hipError_t de = hipGetDevice(&dev); de != hipSuccess)
+        return fail(VA_ERR_HIP, "no HIP device: %s", hipGetErrorString(de));
     NoallocDevice *d = nullptr;
     if (va_status st = noalloc_device(dev, &d)) return st;
     int32_t *counters = reinterpret_cast<int32_t *>(d->ptr);

@@ -12,6 +12,9 @@ bool kernels_lotka_volterra_small(int n, int tableau, int adaptive, KernelSet *o
 bool kernels_lotka_volterra_10(int tableau, int adaptive, int instances_per_wave,
                                KernelSet *out);
 bool kernels_lotka_volterra_shared(int n, int tableau, int adaptive, KernelSet *out);
+bool kernels_lotka_volterra_rows_a(int n, int tableau, int adaptive, KernelSet *out);
+bool kernels_lotka_volterra_rows_b(int n, int tableau, int adaptive, KernelSet *out);
+bool kernels_lotka_volterra_rows_c(int n, int tableau, int adaptive, KernelSet *out);
 
 __global__ void k_plain_finish(double *lambda_out, const double *lambda_in, double *mu,
                                const double *pbar, const int32_t *status, int64_t B, int n,
@@ -84,7 +87,14 @@ bool get_kernels(int system, int n, int tableau, int adaptive, int instances_per
         return n == 2 && one_lane && kernels_oscillator_fixed_k(tableau, adaptive, out);
     case SYS_GLV:
         if (n == 10) return kernels_lotka_volterra_10(tableau, adaptive, instances_per_wave, out);
-        return one_lane && kernels_lotka_volterra_small(n, tableau, adaptive, out);
+        // one lane per instance (instances_per_wave 64): N = 2..5; one group of N
+        // lanes per instance (0 or 64/N, the default): N = 2..32
+        if (instances_per_wave == 64) return kernels_lotka_volterra_small(n, tableau, adaptive, out);
+        if (n < 2 || n > 32 || (instances_per_wave != 0 && instances_per_wave != 64 / n))
+            return false;
+        if (n <= 9) return kernels_lotka_volterra_rows_a(n, tableau, adaptive, out);
+        if (n <= 20) return kernels_lotka_volterra_rows_b(n, tableau, adaptive, out);
+        return kernels_lotka_volterra_rows_c(n, tableau, adaptive, out);
     case SYS_GLV_SHARED: return kernels_lotka_volterra_shared(n, tableau, adaptive, out);
     default: return false;
     }

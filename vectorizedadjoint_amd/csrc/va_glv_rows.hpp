@@ -892,7 +892,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
     for (int j = 0; j < NN; ++j) a.st.pbar[(int64_t)(NN * (i + 1) + j) * B + b] = Abar[j];
 }
 
-template <int NN, class Tab, bool Ad>
+template <int NN, class Tab, bool Ad, bool WithLong = true>
 struct LaunchGlvRows {
     static hipError_t fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
     {
@@ -959,13 +959,42 @@ struct LaunchGlvRows {
         k->occ_rev = &occ_rev;
         k->slots = &slots;
         k->layout = 1;
-        if constexpr (Ad) {
+        k->s_rev = Tab::S;
+        k->s_fwd = Tab::SF;
+        if constexpr (Ad && WithLong) {
             k->long_fused = &lng;
             k->long_fwd = &lng_fwd;
             k->long_rev = &lng_rev;
         }
     }
 };
+
+// Row-per-lane kernels only (no one-lane-per-instance kernels, whose per-lane
+// copy of A does not fit in registers for larger N, and no long-instance kernels:
+// solves then run unsplit): the adaptive Cash-Karp 5(4), DOPRI5 and Fehlberg
+// 7(8) loops and the fixed-step Euler and RK4 loops.
+template <int NN>
+inline bool pick_glv_rows_core(int tableau, int adaptive, KernelSet *out)
+{
+    *out = KernelSet{};
+    out->n = NN;
+    out->np = LotkaVolterra<NN>::P;
+    out->flop_rhs = LotkaVolterra<NN>::FLOP_RHS;
+    out->flop_vjp = LotkaVolterra<NN>::FLOP_VJP;
+    if (adaptive) {
+        switch (tableau) {
+        case TAB_CK54: LaunchGlvRows<NN, CashKarp54, true, false>::apply(out); return true;
+        case TAB_DOPRI5: LaunchGlvRows<NN, Dopri5, true, false>::apply(out); return true;
+        case TAB_F78: LaunchGlvRows<NN, Fehlberg78, true, false>::apply(out); return true;
+        default: return false;
+        }
+    }
+    switch (tableau) {
+    case TAB_EULER: LaunchGlvRows<NN, Euler, false, false>::apply(out); return true;
+    case TAB_RK4: LaunchGlvRows<NN, RK4, false, false>::apply(out); return true;
+    default: return false;
+    }
+}
 
 template <int NN>
 inline bool pick_glv_rows(int tableau, int adaptive, KernelSet *out)

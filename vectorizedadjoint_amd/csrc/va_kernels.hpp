@@ -90,8 +90,15 @@ struct ObsStore {
     int32_t slots;         // checkpoints per instance (even)
     int32_t stride_max;    // largest checkpoint spacing; slots * stride_max >= cap
     int32_t *stride;       // [B] spacing each instance ended with
-    double *seg;           // [lanes][stride_max][N+1] (h, y) rows recomputed by the reverse
+    double *seg;           // [lanes][seg_rows][N+1] (h, y) rows recomputed by the reverse
     int32_t *obs_row;      // [B][K] stored-point index of each observation
+    // two levels: a segment longer than seg_rows is replayed once to store level-2
+    // checkpoints every `sub` steps (sub <= seg_rows) in ck2, then recomputed
+    // sub-segment by sub-segment into seg
+    int32_t levels = 1;
+    int32_t seg_rows = 0;  // rows of the per-lane row buffer (stride_max with one level)
+    int32_t slots2 = 0;    // level-2 checkpoints per lane
+    double *ck2 = nullptr; // [lanes][slots2][N+3], lanes interleaved like seg
 };
 
 struct ObsFwdArgs {
@@ -152,7 +159,13 @@ struct SharedRevArgs {
     double *gscr;          // [grid][waves][S][64][4] growth products of the step being reversed
 };
 
+// Layout version of KernelSet: kernel tables made in a caller's translation unit
+// (VA_DEVICE_SYSTEM) are accepted only with the library's size and version.
+constexpr int32_t VA_KERNEL_TABLE_VERSION = 1;
+
 struct KernelSet {
+    int32_t table_size = (int32_t)sizeof(KernelSet);
+    int32_t table_version = VA_KERNEL_TABLE_VERSION;
     int n, np, s_rev, s_fwd;
     int flop_rhs, flop_vjp;
     hipError_t (*fwd)(const FwdArgs &, int grid, int block, hipStream_t);

@@ -219,21 +219,23 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 // Reverse
 // ---------------------------------------------------------------------------
 
-// Rebuild rows n0 .. n1-1 ((h_n, y_n) into seg, h_n = t_{n+1} - t_n) from the
-// checkpoint c, replaying
-// the forward's controlled loop with the same observation stops.  Field f of
-// row k is seg[(k (N+1) + f) ss]: the lanes' buffers are interleaved (ss = the
-// grid's thread count), so a wave's stores and loads of one field coalesce.  One trip of
-// the loop is one attempt, whether the lane starts a step or retries one, so
-// the lanes of a wave do not wait for each other's rejected attempts step by
-// step (with 64 lanes some lane rejects in most steps); the values formed are
-// those of the forward.
-template <class Sys, class Tab>
+// Replay `len` accepted steps from the checkpoint c (t, dt to try, observations
+// taken, y), running the forward's controlled loop with the same observation
+// stops, and either (Rows) rebuild their rows (h_n, y_n) into seg, h_n =
+// t_{n+1} - t_n, or (!Rows) store a checkpoint of the same form as c at the start
+// of every `sub`-th step into seg (the level-2 checkpoints of a long segment).
+// Field f of row / checkpoint k is seg[(k W + f) ss] with W = N+1 / N+3: the
+// lanes' buffers are interleaved (ss = the grid's thread count), so a wave's
+// stores and loads of one field coalesce.  One trip of the loop is one attempt,
+// whether the lane starts a step or retries one, so the lanes of a wave do not
+// wait for each other's rejected attempts step by step (with 64 lanes some lane
+// rejects in most steps); the values formed are those of the forward.
+template <class Sys, class Tab, bool Rows = true>
 __device__ void recompute_segment(const typename Sys::Params &q, const SolveParams &sp,
                                   const double *obs_t, int K, const double *c, int32_t len,
-                                  double *seg, int64_t ss)
+                                  double *seg, int64_t ss, int32_t sub = 1)
 {
-    constexpr int N = Sys::N, FS = N + 1;
+    constexpr int N = Sys::N, FS = N + 1, FC = N + 3;
     using R = RK<Sys, Tab>;
     double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
     double t = c[0], dt = c[1], dt_keep = 0.0;
@@ -256,9 +258,18 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
                 if (!is_obs || less_with_sign(t, target, dt)) break;
                 ++kobs;
             }
-            r = seg + (size_t)k * FS * ss;
+            if constexpr (Rows) {
+                r = seg + (size_t)k * FS * ss;
 #pragma unroll
-            for (int i = 0; i < N; ++i) r[(size_t)(1 + i) * ss] = y[i];
+                for (int i = 0; i < N; ++i) r[(size_t)(1 + i) * ss] = y[i];
+            } else if (k % sub == 0) {
+                double *c2 = seg + (size_t)(k / sub) * FC * ss;
+                c2[0] = t;
+                c2[ss] = dt;
+                c2[2 * ss] = (double)kobs;
+#pragma unroll
+                for (int i = 0; i < N; ++i) c2[(size_t)(3 + i) * ss] = y[i];
+            }
             clipped = false;
             dt_keep = dt;
             if (less_with_sign(target, t + dt, dt)) {
@@ -276,7 +287,7 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
         const bool ok = control_step<Tab>(dt, err, dt_next);
         if (ok) {
             const double tn = t + dt;
-            r[0] = stored_step(tn, t);
+            if constexpr (Rows) r[0] = stored_step(tn, t);
             t = tn;
 #pragma unroll
             for (int i = 0; i < N; ++i) y[i] = ynew[i];
@@ -352,11 +363,9 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
             const int32_t stride = a.st.stride[b];
             const double *ck = a.st.ck + (size_t)b * (size_t)a.st.slots * FC;
             const int32_t nseg = (cnt + stride - 1) / stride;
-            for (int32_t j = nseg - 1; j >= 0; --j) {
-                const int32_t n0 = j * stride;
-                const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
-                recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, ck + (size_t)j * FC, len, seg,
-                                            ss);
+            // reverse `len` rows from checkpoint c; the first is point n0
+            auto reverse_rows = [&](const double *c, int32_t n0, int32_t len) {
+                recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, c, len, seg, ss);
                 for (int32_t k = len - 1; k >= 0; --k) {
                     const double *r = seg + (size_t)k * FS * ss;
                     double yk[N];
@@ -364,6 +373,29 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
                     for (int i = 0; i < N; ++i) yk[i] = r[(size_t)(1 + i) * ss];
                     back_prop_step<Sys, Tab>(q, r[0], yk, lam, pbar);
                     kobs = obs_jumps<N>(orow, a.obs_y, kobs, n0 + k, yk, lam);
+                }
+            };
+            for (int32_t j = nseg - 1; j >= 0; --j) {
+                const int32_t n0 = j * stride;
+                const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
+                if (len <= a.st.seg_rows) {
+                    reverse_rows(ck + (size_t)j * FC, n0, len);
+                    continue;
+                }
+                // two levels: level-2 checkpoints every `sub` steps of the segment,
+                // then each sub-segment, last first (sub <= seg_rows and
+                // ceil(len / sub) <= slots2 by the sizing in va_create)
+                const int32_t sub = (len + a.st.slots2 - 1) / a.st.slots2;
+                double *ck2 = a.st.ck2 + gtid;
+                recompute_segment<Sys, Tab, false>(q, sp, a.obs_t, K, ck + (size_t)j * FC, len,
+                                                   ck2, ss, sub);
+                for (int32_t i = (len - 1) / sub; i >= 0; --i) {
+                    double c2[FC];
+#pragma unroll
+                    for (int f = 0; f < FC; ++f) c2[f] = ck2[(size_t)(i * FC + f) * ss];
+                    const int32_t m0 = i * sub;
+                    const int32_t leni = (len - m0 < sub) ? len - m0 : sub;
+                    reverse_rows(c2, n0 + m0, leni);
                 }
             }
         }

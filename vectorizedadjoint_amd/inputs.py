@@ -106,6 +106,12 @@ WORKLOADS = {
         "DOPRI5 tol 1e-8, J = sum_k |y(t_k) - y_obs(t_k)|^2 at t_k = 20k/50 (k=1..50), "
         "observations from a seeded ground-truth mu, B=262144, 4 GB trajectory budget",
         extra={"n_obs": 50, "budget_bytes": 4 << 30}),
+    "brusselator": Workload(
+        "brusselator", 6, K.SYS_USER, 2, 2, K.TAB_DOPRI5, 0.0, 20.0, 1e-3, 1e-8, 4096,
+        K.COST_SQUARED_NORM,
+        "Brusselator x0'=a+x0^2 x1-(b+1)x0, x1'=b x0-x0^2 x1 (a system of the caller's, "
+        "compiled outside the library), a~U(0.5,1.5), b~U(1.5,3.5), y0~U(0.5,2)^2, "
+        "t in [0,20], DOPRI5 tol 1e-8, J=|y(T)|^2, B=4096"),
     "glv128_shared": Workload(
         "glv128_shared", 4, K.SYS_LOTKA_VOLTERRA_SHARED, 128, 128, K.TAB_CASH_KARP54, 0.0, 50.0,
         1e-3, 1e-8, 16384, K.COST_SUM,
@@ -150,6 +156,13 @@ def make_inputs(w: Workload | str, B: int | None = None) -> dict:
         x0 = 0.1 + 0.9 * uniform(s, 3, B, N)
         p = np.concatenate([r, A.reshape(B, N * N)], axis=1)
         return {"y0": np.ascontiguousarray(x0.T), "p": np.ascontiguousarray(p.T),
+                "p_shared": None}
+    if w.system == K.SYS_USER and w.name == "brusselator":
+        u = uniform(s, 0, B, 2)
+        a = 0.5 + u[:, 0]
+        b = 1.5 + 2.0 * u[:, 1]
+        y0 = 0.5 + 1.5 * uniform(s, 1, B, 2)
+        return {"y0": np.ascontiguousarray(y0.T), "p": np.ascontiguousarray(np.stack([a, b])),
                 "p_shared": None}
     if w.system == K.SYS_LOTKA_VOLTERRA_SHARED:
         A = 0.05 * normal(s, 1, 1, N * N).reshape(N, N)
