@@ -64,8 +64,11 @@ def _compare(g, o, exact=True, grad_tol=TOL):
           f" yT {e_y:.2e} J {e_J:.2e} dJdy0 {e_y0:.2e} dJdp {e_p:.2e}", flush=True)
     assert frac >= 0.999, f"step sequences equal on only {frac:.5f} of instances"
     if exact:
+        # bit patterns, so that a zero of the other sign counts as a difference
         for k in ("yT", "J", "dJdy0", "dJdp"):
-            assert np.array_equal(g[k][..., m], o[k][..., m]), f"{k} differs from the oracle"
+            gk = np.ascontiguousarray(g[k][..., m], dtype=np.float64).view(np.uint64)
+            ok_ = np.ascontiguousarray(o[k][..., m], dtype=np.float64).view(np.uint64)
+            assert np.array_equal(gk, ok_), f"{k} differs from the oracle"
     assert e_y <= TOL and e_J <= TOL
     assert e_y0 <= grad_tol and e_p <= grad_tol
     assert np.all(np.isnan(g["dJdp"][:, ~ok]))
@@ -219,6 +222,35 @@ def test_fixed_step_euler_matches_oracle(name, ipw):
     assert (o["status"] == 0).all() and (o["n_acc"] > 400).all()
     g = _solve_gpu(w, inp, MODE_PERSISTENT, max_steps=4096, instances_per_wave=ipw)
     _compare(g, o)
+
+
+def test_glv_signed_zeros_match_oracle():
+    """Lotka-Volterra with exact zeros: growth rates r = -0.0, species at 0.0 (whose
+    products A_ij x_j are zeros of either sign), a row of A of -0.0, and an
+    instance that is zero throughout.  The row kernels start each growth sum at
+    the first product and take r as r + 0.0; the reference sums from 0.0 and adds
+    r.  Outputs are compared bit for bit, signs of zeros included."""
+    import dataclasses
+    from vectorizedadjoint_amd import inputs as I, MODE_PERSISTENT, MODE_PLAIN
+    w = dataclasses.replace(I.WORKLOADS["glv10"], t1=10.0)
+    B, N = 256, 10
+    inp = I.make_inputs(w, B)
+    y0, p = inp["y0"].copy(), inp["p"].copy()
+    p[:N, 0:64] = -0.0                          # r = -0
+    y0[[0, 3, 7], 32:128] = 0.0                 # extinct species
+    y0[:, 160:192] = 0.0                        # everything extinct ...
+    p[:N, 160:176] = -0.0                       # ... with r = -0 on half of them
+    for k in range(192, 224):                   # a row of A of -0.0
+        i = k % N
+        p[N * (i + 1):N * (i + 2), k] = -0.0
+    p[N:, 224:240] = -np.abs(p[N:, 224:240])    # all interactions negative ...
+    y0[[1, 2], 224:240] = 0.0                   # ... with extinct species
+    inp = dict(inp, y0=y0, p=p)
+    o = run_oracle(w, inp, max_steps=1024)
+    assert (o["status"] == 0).sum() > 200
+    for mode in (MODE_PERSISTENT, MODE_PLAIN):
+        g = _solve_gpu(w, inp, mode, max_steps=1024)
+        _compare(g, o)
 
 
 @pytest.mark.parametrize("n,tableau", [(7, "cash_karp54"), (16, "cash_karp54"),

@@ -62,10 +62,22 @@ __device__ __forceinline__ void gather_vec(const double *slab, int gbase, double
     }
 }
 
-// Maximum of a group's error ratios.  Every value is >= +0 or NaN; fmax is
-// exact, ignores NaN operands like the sequential (m < e) ? e : m fold, and is
-// order-independent on such values, so a balanced tree gives the same result
-// with a dependency depth of log2(N).
+// v_max_f64 without the quieting of its operands that fmax() adds for values
+// the compiler cannot prove quiet (LDS loads and DPP moves): the error ratios
+// are results of arithmetic, so a NaN among them is already quiet, and
+// v_max_f64 returns the other operand for a quiet NaN.
+__device__ __forceinline__ double max_quiet(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Maximum of a group's error ratios.  Every value is >= +0 or a quiet NaN; the
+// maximum is exact, ignores NaN operands like the sequential (m < e) ? e : m
+// fold, and is order-independent on such values, so a balanced tree gives the
+// same result with a dependency depth of log2(N); the fold starts at 0, which
+// an all-NaN group returns.
 template <int NN>
 __device__ __forceinline__ double group_max(const double (&v)[NN])
 {
@@ -75,8 +87,8 @@ __device__ __forceinline__ double group_max(const double (&v)[NN])
 #pragma unroll
     for (int w = 1; w < NN; w *= 2)
 #pragma unroll
-        for (int j = 0; j + w < NN; j += 2 * w) t[j] = fmax(t[j], t[j + w]);
-    return fmax(t[0], 0.0);
+        for (int j = 0; j + w < NN; j += 2 * w) t[j] = max_quiet(t[j], t[j + w]);
+    return max_quiet(t[0], 0.0);
 }
 
 // Long-running instances set the job's critical path: a wave holding one gets
@@ -177,15 +189,26 @@ __device__ __forceinline__ void xchg(double *slab, int lane, int gbase, double x
 // g_i = r_i + sum_j A_ij x_j in the reference functor's order
 // (examples/GeneralizedLotkaVolterra/main.cpp:112-117): sum = 0, sum += A_ij x_j
 // for j = 0..N-1 with each product rounded, then r_i + sum.
+//
+// The sum starts at the first product instead of at 0.0 + p_0: one add fewer
+// per row, with the same result when r is taken as r + 0.0 (growth_rate()).
+// 0.0 + p equals p unless p = -0, and a sum of rounded products started at
+// +0.0 is never -0, so the two sums differ only when every product is -0 (+0
+// against -0); then r + (+0) and r' + (-0) with r' = r + 0.0 agree for every r,
+// and where they do not, r' + s = r + s as well.
 template <int NN>
-__device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r,
+__device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r0,
                                              const double (&x)[NN])
 {
-    double sum = 0.0;
+    double sum = Arow[0] * x[0];
 #pragma unroll
-    for (int j = 0; j < NN; ++j) sum = sum + Arow[j] * x[j];
-    return r + sum;
+    for (int j = 1; j < NN; ++j) sum = sum + Arow[j] * x[j];
+    return r0 + sum;
 }
+
+// The growth rate as growth_row takes it: r + 0.0 (-0 becomes +0, every other
+// value is unchanged).
+__device__ __forceinline__ double growth_rate(double r) { return r + 0.0; }
 
 // xi_j = w_j g_j + sum_k u_k A_kj, split the same way.
 template <int NN>
@@ -270,7 +293,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 if (b >= q.hi) break;
                 if (a.order) b = a.order[b];
             }
-            r = a.p[(int64_t)i * B + b];
+            r = growth_rate(a.p[(int64_t)i * B + b]);
 #pragma unroll
             for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
             y = a.y[(int64_t)i * B + b];
@@ -484,7 +507,7 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                 if (a.cost && i == 0) a.J[b] = qnan;
                 continue;
             }
-            r = a.p[(int64_t)i * B + b];
+            r = growth_rate(a.p[(int64_t)i * B + b]);
 #pragma unroll
             for (int j = 0; j < NN; ++j) {
                 Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
@@ -707,7 +730,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_fwd(PlainFwdArgs a)
     const int gbase = grp * NN;
     const int32_t step = a.step;
     double v[NN], Arow[NN], K[Tab::SF];
-    const double r = a.p[(int64_t)i * B + b];
+    const double r = growth_rate(a.p[(int64_t)i * B + b]);
 #pragma unroll
     for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
     double y = a.y[(int64_t)i * B + b];
@@ -839,7 +862,7 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
     const int gbase = grp * NN;
     double *uslab = reinterpret_cast<double *>(usl2[wv]);
     double v[NN], Arow[NN], Acol[NN];
-    const double r = a.p[(int64_t)i * B + b];
+    const double r = growth_rate(a.p[(int64_t)i * B + b]);
 #pragma unroll
     for (int j = 0; j < NN; ++j) {
         Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
