@@ -242,6 +242,38 @@ __device__ __forceinline__ QueueRange queue_range(int64_t B, const int32_t *spli
     return q;
 }
 
+// The tableau's coefficients as values in registers.  With Opaque the compiler
+// sees them as unknown values formed once per kernel, so it keeps them in
+// vector registers instead of forming each 64-bit constant in scalar registers
+// on every use (two scalar moves each).  The values are the tableau's.
+template <class Tab, bool Opaque>
+struct TabRegs {
+    double a[Tab::SF][Tab::SF], b[Tab::S], db[Tab::SF];
+    __device__ __forceinline__ TabRegs()
+    {
+#pragma unroll
+        for (int m = 0; m < Tab::SF; ++m)
+#pragma unroll
+            for (int j = 0; j < Tab::SF; ++j) {
+                a[m][j] = Tab::A[m][j];
+                if constexpr (Opaque)
+                    if (j < m && Tab::A[m][j] != 0.0) asm volatile("" : "+v"(a[m][j]));
+            }
+#pragma unroll
+        for (int j = 0; j < Tab::S; ++j) {
+            b[j] = Tab::B[j];
+            if constexpr (Opaque)
+                if (Tab::B[j] != 0.0) asm volatile("" : "+v"(b[j]));
+        }
+#pragma unroll
+        for (int j = 0; j < Tab::SF; ++j) {
+            db[j] = Tab::HAS_ERR ? Tab::DB[j] : 0.0;
+            if constexpr (Opaque && Tab::HAS_ERR)
+                if (Tab::DB[j] != 0.0) asm volatile("" : "+v"(db[j]));
+        }
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Forward sweep (persistent, refill per instance group)
 // ---------------------------------------------------------------------------
@@ -276,6 +308,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
     double *rows = nullptr;
     double v[NN];
     const QueueRange q = queue_range(B, a.split, a.part);
+    const TabRegs<Tab, OneShot> tc;
 
     for (;;) {
         if constexpr (OneShot) __builtin_amdgcn_s_setprio(3);
@@ -318,7 +351,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 finish = true;
             } else {
                 double *row = rows + (size_t)count * F;
-                if (i == 0) row[0] = t;
+                row[0] = t;
                 row[2 + i] = y;
                 if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
                 if ((!Tab::FSAL && !kSpec) || !have_k0) {
@@ -337,20 +370,20 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 double X = y;
 #pragma unroll
                 for (int j = 0; j < Tab::SF; ++j)
-                    if (j < m && Tab::A[m][j] != 0.0) X = X + (Tab::A[m][j] * h) * K[j];
+                    if (j < m && Tab::A[m][j] != 0.0) X = X + (tc.a[m][j] * h) * K[j];
                 xchg<OneShot, NN>(slab, lane, gbase, X, v);
                 K[m] = X * growth_row<NN>(Arow, r, v);
             }
             ynew = y;
 #pragma unroll
             for (int j = 0; j < Tab::S; ++j)
-                if (Tab::B[j] != 0.0) ynew = ynew + (Tab::B[j] * h) * K[j];
+                if (Tab::B[j] != 0.0) ynew = ynew + (tc.b[j] * h) * K[j];
             if constexpr (Adaptive) {
                 constexpr int j0 = ErrFirst<Tab>::value();
-                xerr = (Tab::DB[j0] * h) * K[j0];
+                xerr = (tc.db[j0] * h) * K[j0];
 #pragma unroll
                 for (int j = j0 + 1; j < Tab::SF; ++j)
-                    if (Tab::DB[j] != 0.0) xerr = xerr + (Tab::DB[j] * h) * K[j];
+                    if (Tab::DB[j] != 0.0) xerr = xerr + (tc.db[j] * h) * K[j];
                 const double den = sp.atol + sp.rtol * (fabs(y) + h * fabs(K[0]));
                 double k0n = 0.0, ve[NN];
                 if constexpr (kSpec) {
@@ -370,7 +403,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                     ++rej;
                 } else {
                     const double tn = t + dt;
-                    if (i == 0) rows[(size_t)count * F + 1] = stored_step(tn, t);
+                    rows[(size_t)count * F + 1] = stored_step(tn, t);
                     t = tn;
                     y = ynew;
                     if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
@@ -386,7 +419,7 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 }
             } else {
                 const double tn = sp.t0 + (double)(count + 1) * sp.dt0;
-                if (i == 0) rows[(size_t)count * F + 1] = stored_step(tn, t);
+                rows[(size_t)count * F + 1] = stored_step(tn, t);
                 y = ynew;
                 if constexpr (Tab::FSAL) K[0] = K[Tab::SF - 1];
                 ++count;
