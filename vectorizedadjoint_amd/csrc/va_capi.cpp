@@ -399,11 +399,12 @@ bool split_solve(const va_handle *h)
     return h->side && reuse_order(h, h->lanes_fwd);
 }
 
-// Blocks of a persistent kernel beside the long-instance kernel: all of its
-// grid but the room of VA_LONG_MAX waves of its own size, which the long
-// instances hold.  (GLV N=10, B=65536: a solve takes 2.46 ms so, 2.68 ms with
-// two thirds of the rest; a smaller share speeds up the long instances less
-// than it slows down the bulk.)
+// Blocks of a persistent kernel beside a long-instance kernel: all of its
+// grid but the room of VA_LONG_MAX waves of its own size.  (GLV N=10,
+// B=65536: a solve took 2.46 ms so, 2.68 ms with two thirds of the rest; a
+// smaller share speeds up the long instances less than it slows down the
+// bulk.  Room for both waves of each of the solve's long-instance blocks made
+// the bulk's reverse 9% slower.)
 int bulk_grid(const va_handle *h, int64_t lanes)
 {
     const int per_block = h->ks.slots(h->block);
@@ -538,7 +539,8 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.split = h->counters + 5;
     ra.part = 1;
     if ((st = fork_side(h, s))) return st;
-    VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX / 4, 256, s));
+    // one block of two waves per long instance
+    VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 128, s));
     VA_HIP(h->ks.fwd(fa, bulk_grid(h, h->lanes_fwd), h->block, h->side));
     VA_HIP(h->ks.rev(ra, bulk_grid(h, h->lanes_rev), h->block, h->side));
     if ((st = join_side(h, s))) return st;

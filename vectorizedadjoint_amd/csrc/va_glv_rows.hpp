@@ -702,23 +702,233 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 
 // ---------------------------------------------------------------------------
 // Instances expected to run long: queue positions [0, *split) of the forward
-// order, one instance per wave (group 0).  Each wave runs its instance's
-// forward and then at once its reverse, so it never waits for another
-// instance; the rest of the batch (positions from *split) runs in the
-// persistent kernels on another stream meanwhile.
+// order, one block of two waves per instance.  Wave 0 runs the instance's
+// forward and then at once the backward of its reverse, so it never waits for
+// another instance; the rest of the batch (positions from *split) runs in the
+// persistent kernels on another stream meanwhile.  During the reverse, wave 1
+// forms the stage values of each stored step -- the stage inputs X_m and the
+// growth rates g_m from (h_n, y_n) with h_n = t_{n+1} - t_n, the operations of
+// the recompute in glv_rows_rev_body -- a few steps ahead of wave 0 into a ring
+// of LDS slots, so wave 0's dependent chain holds the backward alone
+// (backpropagation.hpp:104-153).  The two waves meet through two counters in
+// LDS: the steps staged and the steps whose slot wave 0 has read.
 // ---------------------------------------------------------------------------
+constexpr int VA_STAGE_RING = 8;         // steps staged ahead of the backward
+constexpr int VA_SPIN_LIMIT = 1 << 22;   // polls of a ring counter (about 0.1 s) before giving up
+// LDS a long-instance block reserves beyond its own use (its own is ~13 KB), so
+// that no two such blocks share a CU's 160 KB: each long forward then has a
+// SIMD without another long wave (two long waves on one SIMD each ran ~40%
+// slower), and the persistent kernels' blocks (<= 10 KB) still fit beside it.
+constexpr int VA_LONG_LDS_PAD = 68 * 1024;
+
 template <int NN, class Tab>
-__global__ void __launch_bounds__(256) k_glv_rows_long(FwdArgs fa, RevArgs ra)
+struct StageRing {
+    double2 xg[VA_STAGE_RING][Tab::S][NN];  // (X_m,i, g_m,i) of a staged step
+    double h[VA_STAGE_RING];                // its h_n
+    int avail;                              // steps staged (wave 1)
+    int consumed;                           // steps whose slot wave 0 has read
+    int abort;                              // a wait gave up: both waves stop
+};
+
+__device__ __forceinline__ int lds_read_volatile(const int *p) { return *(const volatile int *)p; }
+__device__ __forceinline__ void lds_write_volatile(int *p, int v) { *(volatile int *)p = v; }
+
+// Wait until *ctr > k.  Every wait ends: it returns false when the other wave
+// gave up, and gives up itself (setting abort) after VA_SPIN_LIMIT polls.
+template <class Ring>
+__device__ __forceinline__ bool wait_above(Ring &R, const int *ctr, int k)
 {
+    for (int spins = 0; spins < VA_SPIN_LIMIT; ++spins) {
+        if (lds_read_volatile(ctr) > k) return true;
+        if (lds_read_volatile(&R.abort)) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    lds_write_volatile(&R.abort, 1);
+    return false;
+}
+
+// Wave 1: stage values of steps n_acc-1 .. 0 into the ring.
+template <int NN, class Tab>
+__device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, StageRing<NN, Tab> &R)
+{
+    constexpr int S = Tab::S, F = NN + 2;
+    const int i = threadIdx.x & 63;
+    if (i >= NN) return;
+    __builtin_amdgcn_s_setprio(3);
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int32_t nacc = a.n_acc[b];
+    if (a.status[b] != LANE_OK || nacc <= 0) return;
+    const double r = growth_rate(a.p[(int64_t)i * B + b]);
+    double Arow[NN], v[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+    const double *rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+    const TabRegs<Tab, true> tc;
+    // (h_n, y_n,i) of the next two steps, loaded ahead of use
+    int n = nacc - 1;
+    double h0 = rows[(size_t)n * F + 1], y0 = rows[(size_t)n * F + 2 + i], h1 = 0.0, y1 = 0.0;
+    if (n >= 1) {
+        h1 = rows[(size_t)(n - 1) * F + 1];
+        y1 = rows[(size_t)(n - 1) * F + 2 + i];
+    }
+    for (int k = 0; k < nacc; ++k, --n) {
+        const double h = h0, y = y0;
+        h0 = h1;
+        y0 = y1;
+        if (n >= 2) {
+            h1 = rows[(size_t)(n - 2) * F + 1];
+            y1 = rows[(size_t)(n - 2) * F + 2 + i];
+        }
+        // the slot of step k - RING must have been read by wave 0
+        if (k >= VA_STAGE_RING && !wait_above(R, &R.consumed, k - VA_STAGE_RING)) return;
+        const int slot = k % VA_STAGE_RING;
+        double Kn[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            double x = y;
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                if (j < m && Tab::A[m][j] != 0.0) x = x + (tc.a[m][j] * h) * Kn[j];
+            bcast_group0<NN>(x, v);
+            const double g = growth_row<NN>(Arow, r, v);
+            Kn[m] = x * g;
+            R.xg[slot][m][i] = make_double2(x, g);
+        }
+        R.h[slot] = h;
+        // the slot's writes complete before the count that publishes them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        lds_write_volatile(&R.avail, k + 1);
+    }
+}
+
+// Wave 0 after the forward: terminal cost, then the backward of steps
+// n_acc-1 .. 0 from the staged values (the backward of glv_rows_rev_body).
+template <int NN, class Tab>
+__device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
+                                                  StageRing<NN, Tab> &R)
+{
+    constexpr int S = Tab::S, P = NN + NN * NN;
+    const int i = threadIdx.x & 63;
+    if (i >= NN) return;
+    __builtin_amdgcn_s_setprio(3);
+    const int64_t B = a.sp.B;
+    const double qnan = __builtin_nan("");
+    if (a.status[b] != LANE_OK) {
+        a.lambda[(int64_t)i * B + b] = qnan;
+        a.mu[(int64_t)i * B + b] = qnan;
+        for (int j = 0; j < NN; ++j) a.mu[(int64_t)(NN * (i + 1) + j) * B + b] = qnan;
+        if (a.cost) a.J[b] = qnan;
+        return;
+    }
+    const TabRegs<Tab, true> tc;
+    double Acol[NN], Abar[NN], v[NN], vu[NN], rbar = 0.0, lam;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        Acol[j] = a.p[(int64_t)(NN * (j + 1) + i) * B + b];
+        Abar[j] = 0.0;
+    }
+    if (a.cost) {
+        const double yT = a.yT[(int64_t)i * B + b];
+        // terminal cost: J = fold over i in order, lambda_i from y_i(T)
+        bcast_group0<NN>(yT, v);
+        double J = 0.0;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) {
+            if (a.cost == COST_SUM) J = J + v[j];
+            else if (a.cost == COST_SQUARED_NORM) J = fma(v[j], v[j], J);
+            else J = fma(a.cost_w[j], v[j], J);
+        }
+        if (a.cost == COST_SUM) lam = 1.0;
+        else if (a.cost == COST_SQUARED_NORM) lam = 2.0 * yT;
+        else lam = a.cost_w[i];
+        a.J[b] = J;
+    } else {
+        lam = a.lambda[(int64_t)i * B + b];
+    }
+    const int32_t nacc = a.n_acc[b];
+    for (int k = 0; k < nacc; ++k) {
+        if (!wait_above(R, &R.avail, k)) {
+            lam = rbar = qnan;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) Abar[j] = qnan;
+            break;
+        }
+        const int slot = k % VA_STAGE_RING;
+        double X[S], G[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            const double2 xg = R.xg[slot][m][i];
+            X[m] = xg.x;
+            G[m] = xg.y;
+        }
+        const double h = R.h[slot];
+        // the slot's values are in registers: release it to wave 1
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        lds_write_volatile(&R.consumed, k + 1);
+        double Kb[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) Kb[m] = (tc.b[m] * h) * lam;
+        double ybar = lam;
+#pragma unroll
+        for (int ii = 0; ii < S; ++ii) {
+            const int m = S - 1 - ii;
+            const double w = Kb[m];
+            const double xown = X[m];
+            const double u = w * xown;
+            const double wg = w * G[m];
+            bcast_group0<NN>(u, vu);
+            const double xi = adjoint_col<NN>(Acol, wg, vu);
+            rbar = rbar + u;
+            // column i of dJ/dA: dA_ki += u_k X_i, each product rounded before it is added
+#pragma unroll
+            for (int kk = 0; kk < NN; ++kk) Abar[kk] = Abar[kk] + vu[kk] * xown;
+            ybar = ybar + xi;
+            // Kbar_j += (xi a_mj) h (detail/backpropagation.hpp:215)
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                if (j < m && Tab::A[m][j] != 0.0) Kb[j] = Kb[j] + (xi * tc.a[m][j]) * h;
+        }
+        lam = ybar;
+    }
+    a.lambda[(int64_t)i * B + b] = lam;
+    if (a.cost) {
+        a.mu[(int64_t)i * B + b] = 0.0 + rbar;
+#pragma unroll
+        for (int kk = 0; kk < NN; ++kk) a.mu[(int64_t)(NN * (kk + 1) + i) * B + b] = 0.0 + Abar[kk];
+    } else {
+        double *m0 = a.mu + (int64_t)i * B + b;
+        *m0 = *m0 + rbar;
+#pragma unroll
+        for (int kk = 0; kk < NN; ++kk) {
+            double *mk = a.mu + (int64_t)(NN * (kk + 1) + i) * B + b;
+            *mk = *mk + Abar[kk];
+        }
+    }
+    (void)P;
+}
+
+template <int NN, class Tab>
+__global__ void __launch_bounds__(128) k_glv_rows_long(FwdArgs fa, RevArgs ra)
+{
+    __shared__ StageRing<NN, Tab> ring;
+    __shared__ double lds_pad[VA_LONG_LDS_PAD / 8];
+    if (fa.sp.B < 0) lds_pad[threadIdx.x] = 0.0;  // never: keeps the reservation
     load_ctrl_tables();
-    const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (w >= *fa.split || w >= fa.sp.B) return;
+    const int64_t w = blockIdx.x;
+    if (w >= *fa.split || w >= fa.sp.B) return;  // the whole block
     const int64_t b = fa.order[w];
-    glv_rows_fwd_body<NN, Tab, true, true>(fa, b);
-    // the forward's rows, y(T) and status (written by other lanes of the group)
-    // become visible to the reverse's loads
-    __threadfence();
-    glv_rows_rev_body<NN, Tab, true>(ra, b);
+    if (threadIdx.x == 0) {
+        ring.avail = 0;
+        ring.consumed = 0;
+        ring.abort = 0;
+    }
+    if (threadIdx.x < 64) glv_rows_fwd_body<NN, Tab, true, true>(fa, b);
+    // the forward's rows, y(T), counts and status become visible to both waves
+    // (the barrier orders the block's memory operations at workgroup scope)
+    __syncthreads();
+    if (threadIdx.x < 64) glv_long_backward<NN, Tab>(ra, b, ring);
+    else glv_long_stage<NN, Tab>(ra, b, ring);
 }
 
 // The same instances in a forward alone (va_forward) and in a reverse alone
