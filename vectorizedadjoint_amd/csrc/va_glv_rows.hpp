@@ -461,6 +461,250 @@ __global__ void __launch_bounds__(256) k_glv_rows_fwd(FwdArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Forward sweep with two rows per lane (N even): N/2 lanes per instance, so a
+// wave holds 64/(N/2) instances (12 for N = 10) instead of 64/N (6).  Each
+// lane forms both of its rows with the operations of glv_rows_fwd_body, and
+// the work every lane of a group repeats -- the step control, the controller's
+// pow, the products (a_mj h), the exchanges and the error maximum -- is spread
+// over twice as many instances.  Same values as glv_rows_fwd_body.
+// ---------------------------------------------------------------------------
+template <int NN>
+struct GlvRows2 {
+    static_assert(NN % 2 == 0, "two rows per lane need an even N");
+    static constexpr int H = NN / 2;      // lanes per instance
+    static constexpr int GPW = 64 / H;    // instance groups per wave
+    static constexpr int WAVES = 4;       // block of 256 threads
+};
+
+// The group's N-vector from the two values of each of its lanes (row 2l and
+// 2l + 1 in lane l), through the wave's LDS slab.
+template <int NN>
+__device__ __forceinline__ void exchange_pairs(double *slab, int gbase, int li, double v0,
+                                               double v1, double (&out)[NN])
+{
+    reinterpret_cast<double2 *>(slab + gbase)[li] = make_double2(v0, v1);
+    __builtin_amdgcn_wave_barrier();
+    gather_vec<NN>(slab, gbase, out);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NN>
+__device__ __forceinline__ void exchange_pairs2(double *slab_a, double *slab_b, int gbase, int li,
+                                                double a0, double a1, double (&out_a)[NN],
+                                                double b0, double b1, double (&out_b)[NN])
+{
+    reinterpret_cast<double2 *>(slab_a + gbase)[li] = make_double2(a0, a1);
+    reinterpret_cast<double2 *>(slab_b + gbase)[li] = make_double2(b0, b1);
+    __builtin_amdgcn_wave_barrier();
+    gather_vec<NN>(slab_a, gbase, out_a);
+    gather_vec<NN>(slab_b, gbase, out_b);
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int NN, class Tab, bool Adaptive>
+__device__ __forceinline__ void glv_rows2_fwd_body(const FwdArgs &a)
+{
+    using L = GlvRows2<NN>;
+    constexpr int F = NN + 2, SF = Tab::SF;
+    __shared__ double2 slab2[L::WAVES][L::GPW * L::H];
+    __shared__ double2 slabb2[L::WAVES][L::GPW * L::H];
+    __shared__ int gidx[L::WAVES][L::GPW];
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane / L::H, li = lane - grp * L::H;
+    if (grp >= L::GPW) return;
+    const int i0 = 2 * li;  // this lane's rows: i0 and i0 + 1
+    double *slab = reinterpret_cast<double *>(slab2[wv]);
+    double *slabb = reinterpret_cast<double *>(slabb2[wv]);
+    const int gbase = grp * NN;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    constexpr bool kSpec = Adaptive && !Tab::FSAL;
+
+    double A[2][NN], r[2], y[2], K[SF][2], ynew[2], xerr[2];
+    double t = 0.0, dt = 0.0;
+    int64_t b = 0;
+    int32_t count = 0, rej = 0, tries = 0, st = LANE_OK;
+    bool new_step = true, have_k0 = false, need = true;
+    double *rows = nullptr;
+    double v[NN];
+    const QueueRange q = queue_range(B, a.split, a.part);
+    const TabRegs<Tab, false> tc;
+
+    for (;;) {
+        if (!a.split) prioritise_long(!need && count + rej > VA_LONG_ATTEMPTS);
+        if (need) {
+            if (li == 0) gidx[wv][grp] = atomicAdd(a.counter, 1);
+            __builtin_amdgcn_wave_barrier();
+            b = q.lo + gidx[wv][grp];
+            __builtin_amdgcn_wave_barrier();
+            if (b >= q.hi) break;
+            if (a.order) b = a.order[b];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const int i = i0 + e;
+                r[e] = growth_rate(a.p[(int64_t)i * B + b]);
+#pragma unroll
+                for (int j = 0; j < NN; ++j) A[e][j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+                y[e] = a.y[(int64_t)i * B + b];
+            }
+            t = sp.t0;
+            dt = sp.dt0;
+            count = rej = tries = 0;
+            st = LANE_OK;
+            new_step = true;
+            have_k0 = false;
+            need = false;
+            rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
+            wait_vmem();
+        }
+        bool finish = false;
+        if (new_step) {
+            const bool more = Adaptive ? less_with_sign(t, sp.t1, dt)
+                                       : less_eq_with_sign(t + sp.dt0, sp.t1, sp.dt0);
+            if (!more) {
+                finish = true;
+            } else if (count >= sp.cap) {
+                st = LANE_MAX_STEPS;
+                finish = true;
+            } else {
+                double *row = rows + (size_t)count * F;
+                row[0] = t;
+                *reinterpret_cast<double2 *>(row + 2 + i0) = make_double2(y[0], y[1]);
+                if (Adaptive && less_with_sign(sp.t1, t + dt, dt)) dt = sp.t1 - t;
+                if ((!Tab::FSAL && !kSpec) || !have_k0) {
+                    exchange_pairs<NN>(slab, gbase, li, y[0], y[1], v);
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) K[0][e] = y[e] * growth_row<NN>(A[e], r[e], v);
+                    have_k0 = true;
+                }
+                tries = 0;
+                new_step = false;
+            }
+        }
+        if (!finish) {
+            const double h = Adaptive ? dt : sp.dt0;
+#pragma unroll
+            for (int m = 1; m < SF; ++m) {
+                double X[2] = {y[0], y[1]};
+#pragma unroll
+                for (int j = 0; j < SF; ++j)
+                    if (j < m && Tab::A[m][j] != 0.0) {
+                        const double ah = tc.a[m][j] * h;
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) X[e] = X[e] + ah * K[j][e];
+                    }
+                exchange_pairs<NN>(slab, gbase, li, X[0], X[1], v);
+#pragma unroll
+                for (int e = 0; e < 2; ++e) K[m][e] = X[e] * growth_row<NN>(A[e], r[e], v);
+            }
+#pragma unroll
+            for (int e = 0; e < 2; ++e) ynew[e] = y[e];
+#pragma unroll
+            for (int j = 0; j < Tab::S; ++j)
+                if (Tab::B[j] != 0.0) {
+                    const double bh = tc.b[j] * h;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) ynew[e] = ynew[e] + bh * K[j][e];
+                }
+            if constexpr (Adaptive) {
+                constexpr int j0 = ErrFirst<Tab>::value();
+                const double dh0 = tc.db[j0] * h;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) xerr[e] = dh0 * K[j0][e];
+#pragma unroll
+                for (int j = j0 + 1; j < SF; ++j)
+                    if (Tab::DB[j] != 0.0) {
+                        const double dh = tc.db[j] * h;
+#pragma unroll
+                        for (int e = 0; e < 2; ++e) xerr[e] = xerr[e] + dh * K[j][e];
+                    }
+                double ratio[2];
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    ratio[e] = fabs(xerr[e]) / (sp.atol + sp.rtol * (fabs(y[e]) + h * fabs(K[0][e])));
+                double k0n[2] = {0.0, 0.0}, ve[NN];
+                if constexpr (kSpec) {
+                    exchange_pairs2<NN>(slab, slabb, gbase, li, ynew[0], ynew[1], v, ratio[0],
+                                        ratio[1], ve);
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) k0n[e] = ynew[e] * growth_row<NN>(A[e], r[e], v);
+                } else {
+                    exchange_pairs<NN>(slabb, gbase, li, ratio[0], ratio[1], ve);
+                }
+                const double err = group_max<NN>(ve);
+                double dt_next;
+                if (!control_step<Tab>(dt, err, dt_next)) {
+                    ++rej;
+                } else {
+                    const double tn = t + dt;
+                    rows[(size_t)count * F + 1] = stored_step(tn, t);
+                    t = tn;
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        y[e] = ynew[e];
+                        if constexpr (Tab::FSAL) K[0][e] = K[SF - 1][e];
+                        if constexpr (kSpec) K[0][e] = k0n[e];
+                    }
+                    ++count;
+                    new_step = true;
+                }
+                dt = dt_next;
+                ++tries;
+                if (tries > sp.max_attempts) {
+                    st = LANE_MAX_FAIL;
+                    finish = true;
+                }
+            } else {
+                const double tn = sp.t0 + (double)(count + 1) * sp.dt0;
+                rows[(size_t)count * F + 1] = stored_step(tn, t);
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    y[e] = ynew[e];
+                    if constexpr (Tab::FSAL) K[0][e] = K[SF - 1][e];
+                }
+                ++count;
+                t = tn;
+                new_step = true;
+            }
+        }
+        if (finish) {
+            if (st == LANE_OK) {
+                double *row = rows + (size_t)count * F;
+                if (li == 0) {
+                    row[0] = t;
+                    row[1] = 0.0;
+                }
+                *reinterpret_cast<double2 *>(row + 2 + i0) = make_double2(y[0], y[1]);
+                exchange_pairs<NN>(slab, gbase, li, fabs(y[0]), fabs(y[1]), v);
+                bool fin = true;
+#pragma unroll
+                for (int j = 0; j < NN; ++j) fin = fin && (v[j] <= 1.7976931348623157e308);
+                if (!fin) st = LANE_NONFINITE;
+            }
+#pragma unroll
+            for (int e = 0; e < 2; ++e) a.y[(int64_t)(i0 + e) * B + b] = y[e];
+            if (li == 0) {
+                a.n_acc[b] = count;
+                a.n_rej[b] = rej;
+                a.status[b] = st;
+                atomicMax(a.max_steps_seen, count);
+            }
+            need = true;
+        }
+    }
+}
+
+template <int NN, class Tab, bool Adaptive>
+__global__ void __launch_bounds__(256) k_glv_rows2_fwd(FwdArgs a)
+{
+    if constexpr (Adaptive) load_ctrl_tables();
+    glv_rows2_fwd_body<NN, Tab, Adaptive>(a);
+}
+
+
+
+// ---------------------------------------------------------------------------
 // Reverse sweep (persistent, refill per instance group), software-pipelined.
 //
 // The reverse of step n needs the stage inputs X_m and growth rates g_m of
@@ -1160,9 +1404,15 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
 
 template <int NN, class Tab, bool Ad, bool WithLong = true>
 struct LaunchGlvRows {
+    // the persistent forward with two rows per lane where N is even and small
+    // enough for both rows' parameters to stay in registers
+    static constexpr bool kPairs = NN % 2 == 0 && NN <= 16;
     static hipError_t fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_glv_rows_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
+        if constexpr (kPairs)
+            hipLaunchKernelGGL((k_glv_rows2_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_glv_rows_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
     static hipError_t rev(const RevArgs &a, int grid, int block, hipStream_t s)
@@ -1200,10 +1450,14 @@ struct LaunchGlvRows {
     static int occ_fwd(int block)
     {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_glv_rows_fwd<NN, Tab, Ad>, block,
-                                                         0) != hipSuccess)
-            return 1;
-        return nb;
+        hipError_t e;
+        if constexpr (kPairs)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_glv_rows2_fwd<NN, Tab, Ad>,
+                                                             block, 0);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_glv_rows_fwd<NN, Tab, Ad>,
+                                                             block, 0);
+        return e == hipSuccess ? nb : 1;
     }
     static int occ_rev(int block)
     {
