@@ -101,6 +101,60 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
     out[idx] = ptraj[((size_t)row * fields + f) * (size_t)B + b];
 }
 
+// Instance-major copies for the Lotka-Volterra row kernels.  The persistent
+// kernels take instances in a data-dependent order, so reading [P][B]
+// parameters fetches a whole cache line for each 8-byte value; an instance's
+// P values read from one contiguous block of 8P bytes instead.  A block
+// transposes a tile of 64 instances through LDS (row stride 65 against bank
+// conflicts) with coalesced reads and writes on both sides.
+constexpr int VA_TILE_B = 64;  // instances per tile
+constexpr int VA_TILE_K = 96;  // parameters per tile: 96 x 65 doubles of LDS
+
+__global__ void __launch_bounds__(256) k_to_instance_major(const double *src, double *dst,
+                                                           int64_t P, int64_t B)
+{
+    __shared__ double tile[VA_TILE_K * 65];
+    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B, k0 = (int64_t)blockIdx.y * VA_TILE_K;
+    const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
+    const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
+    for (int idx = threadIdx.x; idx < kn * VA_TILE_B; idx += blockDim.x) {
+        const int k = idx / VA_TILE_B, bb = idx % VA_TILE_B;
+        if (bb < nb) tile[k * 65 + bb] = src[(k0 + k) * B + b0 + bb];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * kn; idx += blockDim.x) {
+        const int bb = idx / kn, k = idx % kn;
+        dst[(b0 + bb) * P + k0 + k] = tile[k * 65 + bb];
+    }
+}
+
+// dJ/dp sums of the row kernels, instance-major [nout][B][P], into mu
+// [nout][P][B]: 0.0 + sum with a cost, mu + sum without one (the updates the
+// kernels make when they write mu themselves).  blockIdx.z is the seed.
+__global__ void __launch_bounds__(256) k_from_instance_major(const double *src, double *dst,
+                                                             int64_t P, int64_t B, int add)
+{
+    __shared__ double tile[VA_TILE_K * 65];
+    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B, k0 = (int64_t)blockIdx.y * VA_TILE_K;
+    const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
+    const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
+    const double *sp = src + (int64_t)blockIdx.z * B * P;
+    double *dp = dst + (int64_t)blockIdx.z * P * B;
+    for (int idx = threadIdx.x; idx < nb * kn; idx += blockDim.x) {
+        const int bb = idx / kn, k = idx % kn;
+        tile[k * 65 + bb] = sp[(b0 + bb) * P + k0 + k];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kn * VA_TILE_B; idx += blockDim.x) {
+        const int k = idx / VA_TILE_B, bb = idx % VA_TILE_B;
+        if (bb < nb) {
+            double *m = dp + (k0 + k) * B + b0 + bb;
+            const double x = tile[k * 65 + bb];
+            *m = add ? *m + x : 0.0 + x;
+        }
+    }
+}
+
 // Longest-first orders.  The reverse of one instance is a chain of n_acc
 // dependent steps, its forward a chain of n_acc + n_rej attempts; when the long
 // chains start last they set the end of the launch, and with one lane per
@@ -255,6 +309,10 @@ struct va_handle {
     // shared-parameter solves
     int shared_grid = 0;
     double *abar_part = nullptr;
+    // Lotka-Volterra row layout, persistent: instance-major parameters [B][P]
+    // and dJ/dp sums [pbar_nout][B][P]
+    double *pim = nullptr, *pbar_im = nullptr;
+    int pbar_nout = 0;
     double *shared_gscr = nullptr;  // per-workgroup growth products of the shared-A reverse
     // VA_FLAG_GRAPH: va_solve recorded once per argument set, then replayed
     hipStream_t cap_stream = nullptr;
@@ -346,6 +404,8 @@ void release(va_handle *h)
     hipFree(h->obs_stride);
     hipFree(h->obs_row);
     hipFree(h->abar_part);
+    hipFree(h->pim);
+    hipFree(h->pbar_im);
     hipFree(h->shared_gscr);
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->cap_stream) hipStreamDestroy(h->cap_stream);
@@ -430,6 +490,63 @@ va_status join_side(va_handle *h, hipStream_t s)
     return VA_OK;
 }
 
+// The Lotka-Volterra row kernels of a persistent handle read instance-major
+// parameters [B][P] (copied from the caller's [P][B] per call) and store
+// instance-major dJ/dp sums [nout][B][P], applied to the caller's mu afterwards.
+bool instance_major(const va_handle *h) { return h->ks.layout == 1 && h->pim != nullptr; }
+
+va_status to_instance_major(const double *p, double *pim, int64_t P, int64_t B, hipStream_t s)
+{
+    const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)((P + VA_TILE_K - 1) / VA_TILE_K));
+    hipLaunchKernelGGL(k_to_instance_major, grid, dim3(256), 0, s, p, pim, P, B);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
+}
+
+// mu = 0.0 + sum (with a cost) or mu + sum (without one), [nout][P][B]
+va_status from_instance_major(const double *pbar_im, double *mu, int64_t P, int64_t B, int nout,
+                              bool add, hipStream_t s)
+{
+    const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)((P + VA_TILE_K - 1) / VA_TILE_K),
+                    (unsigned)nout);
+    hipLaunchKernelGGL(k_from_instance_major, grid, dim3(256), 0, s, pbar_im, mu, P, B,
+                       add ? 1 : 0);
+    VA_HIP(hipGetLastError());
+    return VA_OK;
+}
+
+va_status params_instance_major(va_handle *h, const double *p, hipStream_t s)
+{
+    return to_instance_major(p, h->pim, h->ks.np, h->pb.batch, s);
+}
+
+template <class Args>
+void use_instance_major(const va_handle *h, Args &a)
+{
+    a.p = h->pim;
+}
+
+// dJ/dp sums of nout seeds go to h->pbar_im (grown here for more seeds than it
+// holds: only va_adjoint with several seeds, which no graph captures).
+va_status pbar_instance_major(va_handle *h, va::RevArgs &a, int nout)
+{
+    const int64_t B = h->pb.batch, P = h->ks.np;
+    if (nout > h->pbar_nout) {
+        hipFree(h->pbar_im);
+        h->pbar_im = nullptr;
+        h->pbar_nout = 0;
+        VA_HIP(hipMalloc((void **)&h->pbar_im, sizeof(double) * (size_t)nout * B * P));
+        h->pbar_nout = nout;
+    }
+    a.mu = h->pbar_im;
+    return VA_OK;
+}
+
+va_status pbar_to_mu(va_handle *h, double *mu, int nout, bool add, hipStream_t s)
+{
+    return from_instance_major(h->pbar_im, mu, h->ks.np, h->pb.batch, nout, add, s);
+}
+
 va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream_t s)
 {
     const int64_t B = h->pb.batch;
@@ -445,6 +562,10 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     a.status = h->status;
     a.counter = h->counters + 0;
     a.max_steps_seen = h->counters + 2;
+    if (instance_major(h)) {
+        if ((st = params_instance_major(h, p, s))) return st;
+        use_instance_major(h, a);
+    }
     const bool reuse = reuse_order(h, h->lanes_fwd);
     a.order = reuse ? h->order_fwd : nullptr;
     const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
@@ -489,6 +610,12 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     a.mu = mu;
     a.nout = nout;
     a.counter = h->counters + 1;
+    const bool im = instance_major(h);
+    if (im) {
+        if ((st = params_instance_major(h, p, s))) return st;
+        use_instance_major(h, a);
+        if ((st = pbar_instance_major(h, a, cost ? 1 : nout))) return st;
+    }
     if (split) {
         a.split = h->counters + 5;
         a.part = 1;
@@ -496,10 +623,12 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
         VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
         VA_HIP(h->ks.rev(a, bulk_grid(h, h->lanes_rev), h->block,
                          h->side));
-        return join_side(h, s);
+        if ((st = join_side(h, s))) return st;
+    } else {
+        const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
+        VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     }
-    const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
-    VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+    if (im) return pbar_to_mu(h, mu, cost ? 1 : nout, cost == 0, s);
     return VA_OK;
 }
 
@@ -538,12 +667,20 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.order = h->order_fwd;
     ra.split = h->counters + 5;
     ra.part = 1;
+    const bool im = instance_major(h);
+    if (im) {
+        if ((st = params_instance_major(h, p, s))) return st;
+        use_instance_major(h, fa);
+        use_instance_major(h, ra);
+        if ((st = pbar_instance_major(h, ra, 1))) return st;
+    }
     if ((st = fork_side(h, s))) return st;
     // one block of two waves per long instance
     VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 128, s));
     VA_HIP(h->ks.fwd(fa, bulk_grid(h, h->lanes_fwd), h->block, h->side));
     VA_HIP(h->ks.rev(ra, bulk_grid(h, h->lanes_rev), h->block, h->side));
     if ((st = join_side(h, s))) return st;
+    if (im && (st = pbar_to_mu(h, dJdp, 1, false, s))) return st;
     return order_by_length(h, 1, s);
 }
 
@@ -934,6 +1071,12 @@ va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_
     }
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));
+        if (ks.layout == 1 && full_store) {
+            // instance-major parameters and dJ/dp sums of the row kernels
+            alloc((void **)&h->pim, sizeof(double) * (size_t)B * ks.np);
+            alloc((void **)&h->pbar_im, sizeof(double) * (size_t)B * ks.np);
+            h->pbar_nout = 1;
+        }
     } else {
         const size_t F = 2 + (size_t)ks.n;  // plain rows (t, h, y)
         alloc((void **)&h->ptraj, sizeof(double) * rows * F * (size_t)B);
@@ -1546,17 +1689,23 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
     const int64_t B = pb->batch;
     const int n = ks.n;
     // scratch layout: 8 counters | n_acc | n_rej | status | rows [B][max_steps+1][N+2]
+    // | for the Lotka-Volterra row kernels, instance-major parameters and dJ/dp
+    // sums [B][P] each
     const size_t cnt_bytes = align256(sizeof(int32_t) * (size_t)B);
     const size_t off_nacc = 256, off_nrej = off_nacc + cnt_bytes, off_stat = off_nrej + cnt_bytes;
     const size_t off_traj = off_stat + cnt_bytes;
     const double traj_bytes =
         8.0 * (double)B * ((double)pb->max_steps + 1.0) * (double)(n + 2);
-    const double need = (double)off_traj + traj_bytes;
+    const bool im = ks.layout == 1;
+    const size_t im_bytes = im ? align256(sizeof(double) * (size_t)B * ks.np) : 0;
+    const size_t off_pim = off_traj + align256((size_t)traj_bytes), off_pbar = off_pim + im_bytes;
+    const double need = (double)off_pbar + (double)im_bytes;
     if (need > (double)VA_NOALLOC_SCRATCH_BYTES)
         return fail(VA_ERR_OUT_OF_MEMORY,
                     "va_solve_noalloc needs %.0f bytes of scratch (B * (max_steps+1) * (N+2) * "
-                    "8 bytes of rows plus 12 bytes per instance) and holds %lld: use fewer "
-                    "instances or a smaller max_steps, or va_create + va_solve",
+                    "8 bytes of rows, 12 bytes per instance and, for Lotka-Volterra, 16 P bytes "
+                    "per instance) and holds %lld: use fewer instances or a smaller max_steps, "
+                    "or va_create + va_solve",
                     need, (long long)VA_NOALLOC_SCRATCH_BYTES);
     if (pb->trajectory_budget_bytes > 0 && traj_bytes > (double)pb->trajectory_budget_bytes)
         return fail(VA_ERR_OUT_OF_MEMORY,
@@ -1572,6 +1721,8 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
     int32_t *nrej = reinterpret_cast<int32_t *>(d->ptr + off_nrej);
     int32_t *stat = reinterpret_cast<int32_t *>(d->ptr + off_stat);
     double *traj = reinterpret_cast<double *>(d->ptr + off_traj);
+    double *pim = im ? reinterpret_cast<double *>(d->ptr + off_pim) : nullptr;
+    double *pbar_im = im ? reinterpret_cast<double *>(d->ptr + off_pbar) : nullptr;
 
     hipStream_t s = (hipStream_t)stream;
     const int per_block = ks.slots(block);
@@ -1600,10 +1751,13 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
     sp.kconst = pb->system_constant;
     sp.max_attempts = pb->max_attempts;
     sp.adaptive = pb->adaptive;
+    if (im) {
+        if (va_status st = to_instance_major(p, pim, ks.np, B, s)) return st;
+    }
     va::FwdArgs a{};
     a.sp = sp;
     a.y = yT;
-    a.p = p;
+    a.p = im ? pim : p;
     a.traj = traj;
     a.n_acc = nacc;
     a.n_rej = nrej;
@@ -1614,7 +1768,7 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
     va::RevArgs r{};
     r.sp = sp;
     r.order = nullptr;
-    r.p = p;
+    r.p = im ? pim : p;
     r.traj = traj;
     r.n_acc = nacc;
     r.status = stat;
@@ -1623,10 +1777,14 @@ va_status va_solve_noalloc(const va_problem *pb, const double *y0, const double 
     r.yT = yT;
     r.J = J;
     r.lambda = dJdy0;
-    r.mu = dJdp;
+    r.mu = im ? pbar_im : dJdp;
     r.nout = 1;
     r.counter = counters + 1;
     VA_HIP(ks.rev(r, blocks_for(lanes_r, per_block), block, s));
+    if (im) {
+        if (va_status st = from_instance_major(pbar_im, dJdp, ks.np, B, 1, cost == 0, s))
+            return st;
+    }
     if (n_accepted || n_rejected || lane_status) {
         hipLaunchKernelGGL(k_copy_counts, dim3(blocks_for(B, 256)), dim3(256), 0, s, n_accepted,
                            n_rejected, lane_status, nacc, nrej, stat, B);

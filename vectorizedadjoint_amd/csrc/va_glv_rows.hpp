@@ -210,6 +210,34 @@ __device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r0
 // value is unchanged).
 __device__ __forceinline__ double growth_rate(double r) { return r + 0.0; }
 
+// The persistent row kernels read the parameters instance-major, [B][P] (the
+// instance's 8(N + N^2) bytes contiguous: the kernels take instances in a
+// data-dependent order, and [P][B] reads would fetch a cache line per value),
+// and store each instance's dJ/dp sums instance-major as they are,
+// [nout][B][P]; the host forms 0.0 + sum (with a cost) or mu + sum (without
+// one, adjointSolve adds to mu) in [nout][P][B] afterwards.
+template <int NN, class Args>
+__device__ __forceinline__ double param(const Args &a, int64_t k, int64_t b)
+{
+    constexpr int64_t P = NN + NN * NN;
+    return a.p[b * P + k];
+}
+
+template <int NN>
+__device__ __forceinline__ void put_pbar(const RevArgs &a, int64_t o, int64_t k, int64_t b,
+                                         double x)
+{
+    constexpr int64_t P = NN + NN * NN;
+    a.mu[(o * a.sp.B + b) * P + k] = x;
+}
+
+// A failed instance's dJ/dp output: NaN.
+template <int NN>
+__device__ __forceinline__ void put_pbar_nan(const RevArgs &a, int64_t o, int64_t k, int64_t b)
+{
+    put_pbar<NN>(a, o, k, b, __builtin_nan(""));
+}
+
 // xi_j = w_j g_j + sum_k u_k A_kj, split the same way.
 template <int NN>
 __device__ __forceinline__ double adjoint_col(const double (&Acol)[NN], double wg,
@@ -326,9 +354,9 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 if (b >= q.hi) break;
                 if (a.order) b = a.order[b];
             }
-            r = growth_rate(a.p[(int64_t)i * B + b]);
+            r = growth_rate(param<NN>(a, i, b));
 #pragma unroll
-            for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+            for (int j = 0; j < NN; ++j) Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
             y = a.y[(int64_t)i * B + b];
             t = sp.t0;
             dt = sp.dt0;
@@ -543,9 +571,9 @@ __device__ __forceinline__ void glv_rows2_fwd_body(const FwdArgs &a)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int i = i0 + e;
-                r[e] = growth_rate(a.p[(int64_t)i * B + b]);
+                r[e] = growth_rate(param<NN>(a, i, b));
 #pragma unroll
-                for (int j = 0; j < NN; ++j) A[e][j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+                for (int j = 0; j < NN; ++j) A[e][j] = param<NN>(a, NN * (i + 1) + j, b);
                 y[e] = a.y[(int64_t)i * B + b];
             }
             t = sp.t0;
@@ -777,18 +805,17 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                 const int nseed = a.cost ? 1 : a.nout;
                 for (int oo = 0; oo < nseed; ++oo) {
                     a.lambda[((int64_t)oo * NN + i) * B + b] = qnan;
-                    a.mu[((int64_t)oo * P + i) * B + b] = qnan;
-                    for (int j = 0; j < NN; ++j)
-                        a.mu[((int64_t)oo * P + NN * (i + 1) + j) * B + b] = qnan;
+                    put_pbar_nan<NN>(a, oo, i, b);
+                    for (int j = 0; j < NN; ++j) put_pbar_nan<NN>(a, oo, NN * (i + 1) + j, b);
                 }
                 if (a.cost && i == 0) a.J[b] = qnan;
                 continue;
             }
-            r = growth_rate(a.p[(int64_t)i * B + b]);
+            r = growth_rate(param<NN>(a, i, b));
 #pragma unroll
             for (int j = 0; j < NN; ++j) {
-                Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
-                Acol[j] = a.p[(int64_t)(NN * (j + 1) + i) * B + b];
+                Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
+                Acol[j] = param<NN>(a, NN * (j + 1) + i, b);
             }
             rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
             o = 0;
@@ -904,21 +931,11 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
             --nr;
         }
         if (!back) {
-            const int64_t lo = (int64_t)o * NN, po = (int64_t)o * P;
+            const int64_t lo = (int64_t)o * NN;
             a.lambda[(lo + i) * B + b] = lam;
-            if (a.cost) {
-                a.mu[(po + i) * B + b] = 0.0 + rbar;
+            put_pbar<NN>(a, o, i, b, rbar);
 #pragma unroll
-                for (int k = 0; k < NN; ++k) a.mu[(po + NN * (k + 1) + i) * B + b] = 0.0 + Abar[k];
-            } else {
-                double *m0 = a.mu + (po + i) * B + b;
-                *m0 = *m0 + rbar;
-#pragma unroll
-                for (int k = 0; k < NN; ++k) {
-                    double *mk = a.mu + (po + NN * (k + 1) + i) * B + b;
-                    *mk = *mk + Abar[k];
-                }
-            }
+            for (int k = 0; k < NN; ++k) put_pbar<NN>(a, o, NN * (k + 1) + i, b, Abar[k]);
             ++o;
             if (a.cost || o >= a.nout) {
                 need = true;
@@ -1003,10 +1020,10 @@ __device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, Stag
     const int64_t B = sp.B;
     const int32_t nacc = a.n_acc[b];
     if (a.status[b] != LANE_OK || nacc <= 0) return;
-    const double r = growth_rate(a.p[(int64_t)i * B + b]);
+    const double r = growth_rate(param<NN>(a, i, b));
     double Arow[NN], v[NN];
 #pragma unroll
-    for (int j = 0; j < NN; ++j) Arow[j] = a.p[(int64_t)(NN * (i + 1) + j) * B + b];
+    for (int j = 0; j < NN; ++j) Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
     const double *rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
     const TabRegs<Tab, true> tc;
     // (h_n, y_n,i) of the next two steps, loaded ahead of use
@@ -1060,8 +1077,8 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
     const double qnan = __builtin_nan("");
     if (a.status[b] != LANE_OK) {
         a.lambda[(int64_t)i * B + b] = qnan;
-        a.mu[(int64_t)i * B + b] = qnan;
-        for (int j = 0; j < NN; ++j) a.mu[(int64_t)(NN * (i + 1) + j) * B + b] = qnan;
+        put_pbar_nan<NN>(a, 0, i, b);
+        for (int j = 0; j < NN; ++j) put_pbar_nan<NN>(a, 0, NN * (i + 1) + j, b);
         if (a.cost) a.J[b] = qnan;
         return;
     }
@@ -1069,7 +1086,7 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
     double Acol[NN], Abar[NN], v[NN], vu[NN], rbar = 0.0, lam;
 #pragma unroll
     for (int j = 0; j < NN; ++j) {
-        Acol[j] = a.p[(int64_t)(NN * (j + 1) + i) * B + b];
+        Acol[j] = param<NN>(a, NN * (j + 1) + i, b);
         Abar[j] = 0.0;
     }
     if (a.cost) {
@@ -1136,20 +1153,9 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
         lam = ybar;
     }
     a.lambda[(int64_t)i * B + b] = lam;
-    if (a.cost) {
-        a.mu[(int64_t)i * B + b] = 0.0 + rbar;
+    put_pbar<NN>(a, 0, i, b, rbar);
 #pragma unroll
-        for (int kk = 0; kk < NN; ++kk) a.mu[(int64_t)(NN * (kk + 1) + i) * B + b] = 0.0 + Abar[kk];
-    } else {
-        double *m0 = a.mu + (int64_t)i * B + b;
-        *m0 = *m0 + rbar;
-#pragma unroll
-        for (int kk = 0; kk < NN; ++kk) {
-            double *mk = a.mu + (int64_t)(NN * (kk + 1) + i) * B + b;
-            *mk = *mk + Abar[kk];
-        }
-    }
-    (void)P;
+    for (int kk = 0; kk < NN; ++kk) put_pbar<NN>(a, 0, NN * (kk + 1) + i, b, Abar[kk]);
 }
 
 template <int NN, class Tab>

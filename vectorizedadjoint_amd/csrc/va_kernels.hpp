@@ -18,7 +18,7 @@ struct SolveParams {
 struct FwdArgs {
     SolveParams sp;
     double *y;             // [N][B] in y0, out y(T)
-    const double *p;       // [P][B]
+    const double *p;       // [P][B]; Lotka-Volterra row kernels: [B][P]
     double *traj;          // [B][cap+1][N+1] rows (t, y)
     int32_t *n_acc, *n_rej, *status;
     int32_t *counter;
@@ -32,6 +32,8 @@ struct FwdArgs {
 //   cost > 0: lambda computed from y(T) (read from yT) and written to dJdy0,
 //             dJdp = sum p_bar, J written.
 //   cost == 0: lambda [nout][N][B] in/out, mu [nout][P][B] += (reference semantics).
+// The Lotka-Volterra row kernels take p as [B][P] and store the dJ/dp sums
+// [nout][B][P] in mu as they are, for either cost (the caller applies them).
 struct RevArgs {
     SolveParams sp;
     const double *p;
