@@ -130,9 +130,12 @@ __global__ void __launch_bounds__(256) k_to_instance_major(const double *src, do
 
 // dJ/dp sums of the row kernels, instance-major [nout][B][P], into mu
 // [nout][P][B]: 0.0 + sum with a cost, mu + sum without one (the updates the
-// kernels make when they write mu themselves).  blockIdx.z is the seed.
+// kernels make when they write mu themselves); with set, only the instances
+// marked there (the others' mu is the long-instance kernel's).  blockIdx.z is
+// the seed.
 __global__ void __launch_bounds__(256) k_from_instance_major(const double *src, double *dst,
-                                                             int64_t P, int64_t B, int add)
+                                                             int64_t P, int64_t B, int add,
+                                                             const int32_t *set)
 {
     __shared__ double tile[VA_TILE_K * 65];
     const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B, k0 = (int64_t)blockIdx.y * VA_TILE_K;
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(256) k_from_instance_major(const double *src, 
     __syncthreads();
     for (int idx = threadIdx.x; idx < kn * VA_TILE_B; idx += blockDim.x) {
         const int k = idx / VA_TILE_B, bb = idx % VA_TILE_B;
-        if (bb < nb) {
+        if (bb < nb && (!set || set[b0 + bb])) {
             double *m = dp + (k0 + k) * B + b0 + bb;
             const double x = tile[k * 65 + bb];
             *m = add ? *m + x : 0.0 + x;
@@ -313,6 +316,7 @@ struct va_handle {
     // and dJ/dp sums [pbar_nout][B][P]
     double *pim = nullptr, *pbar_im = nullptr;
     int pbar_nout = 0;
+    int32_t *pbar_set = nullptr;  // [B]: instances whose sums are in pbar_im
     double *shared_gscr = nullptr;  // per-workgroup growth products of the shared-A reverse
     // VA_FLAG_GRAPH: va_solve recorded once per argument set, then replayed
     hipStream_t cap_stream = nullptr;
@@ -406,6 +410,7 @@ void release(va_handle *h)
     hipFree(h->abar_part);
     hipFree(h->pim);
     hipFree(h->pbar_im);
+    hipFree(h->pbar_set);
     hipFree(h->shared_gscr);
     if (h->graph_exec) hipGraphExecDestroy(h->graph_exec);
     if (h->cap_stream) hipStreamDestroy(h->cap_stream);
@@ -490,9 +495,12 @@ va_status join_side(va_handle *h, hipStream_t s)
     return VA_OK;
 }
 
-// The Lotka-Volterra row kernels of a persistent handle read instance-major
+// The Lotka-Volterra persistent kernels of a handle read instance-major
 // parameters [B][P] (copied from the caller's [P][B] per call) and store
-// instance-major dJ/dp sums [nout][B][P], applied to the caller's mu afterwards.
+// instance-major dJ/dp sums [nout][B][P], applied to the caller's mu afterwards
+// (for the instances they marked, beside a long-instance kernel, which reads
+// and writes the caller's arrays itself).  Beside a long-instance kernel the
+// copies run on the second stream, off the long kernel's path.
 bool instance_major(const va_handle *h) { return h->ks.layout == 1 && h->pim != nullptr; }
 
 va_status to_instance_major(const double *p, double *pim, int64_t P, int64_t B, hipStream_t s)
@@ -503,14 +511,15 @@ va_status to_instance_major(const double *p, double *pim, int64_t P, int64_t B, 
     return VA_OK;
 }
 
-// mu = 0.0 + sum (with a cost) or mu + sum (without one), [nout][P][B]
+// mu = 0.0 + sum (with a cost) or mu + sum (without one), [nout][P][B], for the
+// instances marked in set (all when set is null)
 va_status from_instance_major(const double *pbar_im, double *mu, int64_t P, int64_t B, int nout,
-                              bool add, hipStream_t s)
+                              bool add, hipStream_t s, const int32_t *set = nullptr)
 {
     const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)((P + VA_TILE_K - 1) / VA_TILE_K),
                     (unsigned)nout);
     hipLaunchKernelGGL(k_from_instance_major, grid, dim3(256), 0, s, pbar_im, mu, P, B,
-                       add ? 1 : 0);
+                       add ? 1 : 0, set);
     VA_HIP(hipGetLastError());
     return VA_OK;
 }
@@ -518,12 +527,6 @@ va_status from_instance_major(const double *pbar_im, double *mu, int64_t P, int6
 va_status params_instance_major(va_handle *h, const double *p, hipStream_t s)
 {
     return to_instance_major(p, h->pim, h->ks.np, h->pb.batch, s);
-}
-
-template <class Args>
-void use_instance_major(const va_handle *h, Args &a)
-{
-    a.p = h->pim;
 }
 
 // dJ/dp sums of nout seeds go to h->pbar_im (grown here for more seeds than it
@@ -542,9 +545,24 @@ va_status pbar_instance_major(va_handle *h, va::RevArgs &a, int nout)
     return VA_OK;
 }
 
-va_status pbar_to_mu(va_handle *h, double *mu, int nout, bool add, hipStream_t s)
+va_status pbar_to_mu(va_handle *h, double *mu, int nout, bool add, hipStream_t s,
+                     const int32_t *set = nullptr)
 {
-    return from_instance_major(h->pbar_im, mu, h->ks.np, h->pb.batch, nout, add, s);
+    return from_instance_major(h->pbar_im, mu, h->ks.np, h->pb.batch, nout, add, s, set);
+}
+
+// The bulk's copy-in, instance marks and arguments beside a long-instance
+// kernel, on stream s (the second stream).
+va_status bulk_instance_major(va_handle *h, const double *p, va::RevArgs *r, int nout,
+                              hipStream_t s)
+{
+    va_status st = params_instance_major(h, p, s);
+    if (st || !r) return st;
+    r->p = h->pim;
+    if ((st = pbar_instance_major(h, *r, nout))) return st;
+    if ((st = fill_i32(h->pbar_set, 0, h->pb.batch, s))) return st;
+    r->pbar_set = h->pbar_set;
+    return VA_OK;
 }
 
 va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream_t s)
@@ -562,10 +580,7 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     a.status = h->status;
     a.counter = h->counters + 0;
     a.max_steps_seen = h->counters + 2;
-    if (instance_major(h)) {
-        if ((st = params_instance_major(h, p, s))) return st;
-        use_instance_major(h, a);
-    }
+    const bool im = instance_major(h);
     const bool reuse = reuse_order(h, h->lanes_fwd);
     a.order = reuse ? h->order_fwd : nullptr;
     const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
@@ -574,10 +589,18 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
         a.part = 1;
         if ((st = fork_side(h, s))) return st;
         VA_HIP(h->ks.long_fwd(a, VA_LONG_MAX / 4, 256, s));
-        VA_HIP(h->ks.fwd(a, bulk_grid(h, h->lanes_fwd), h->block,
-                         h->side));
+        va::FwdArgs ab = a;
+        if (im) {
+            if ((st = bulk_instance_major(h, p, nullptr, 0, h->side))) return st;
+            ab.p = h->pim;
+        }
+        VA_HIP(h->ks.fwd(ab, bulk_grid(h, h->lanes_fwd), h->block, h->side));
         if ((st = join_side(h, s))) return st;
     } else {
+        if (im) {
+            if ((st = params_instance_major(h, p, s))) return st;
+            a.p = h->pim;
+        }
         VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     }
     if (reuse) return order_by_length(h, 1, s);
@@ -611,24 +634,26 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     a.nout = nout;
     a.counter = h->counters + 1;
     const bool im = instance_major(h);
-    if (im) {
-        if ((st = params_instance_major(h, p, s))) return st;
-        use_instance_major(h, a);
-        if ((st = pbar_instance_major(h, a, cost ? 1 : nout))) return st;
-    }
+    const int nseed = cost ? 1 : nout;
     if (split) {
         a.split = h->counters + 5;
         a.part = 1;
         if ((st = fork_side(h, s))) return st;
         VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
-        VA_HIP(h->ks.rev(a, bulk_grid(h, h->lanes_rev), h->block,
-                         h->side));
-        if ((st = join_side(h, s))) return st;
-    } else {
-        const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
-        VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+        va::RevArgs ab = a;
+        if (im && (st = bulk_instance_major(h, p, &ab, nseed, h->side))) return st;
+        VA_HIP(h->ks.rev(ab, bulk_grid(h, h->lanes_rev), h->block, h->side));
+        if (im && (st = pbar_to_mu(h, mu, nseed, cost == 0, h->side, h->pbar_set))) return st;
+        return join_side(h, s);
     }
-    if (im) return pbar_to_mu(h, mu, cost ? 1 : nout, cost == 0, s);
+    if (im) {
+        if ((st = params_instance_major(h, p, s))) return st;
+        a.p = h->pim;
+        if ((st = pbar_instance_major(h, a, nseed))) return st;
+    }
+    const int64_t lanes = std::min<int64_t>(h->lanes_rev, B);
+    VA_HIP(h->ks.rev(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
+    if (im) return pbar_to_mu(h, mu, nseed, cost == 0, s);
     return VA_OK;
 }
 
@@ -668,19 +693,19 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.split = h->counters + 5;
     ra.part = 1;
     const bool im = instance_major(h);
-    if (im) {
-        if ((st = params_instance_major(h, p, s))) return st;
-        use_instance_major(h, fa);
-        use_instance_major(h, ra);
-        if ((st = pbar_instance_major(h, ra, 1))) return st;
-    }
     if ((st = fork_side(h, s))) return st;
     // one block of two waves per long instance
     VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 128, s));
-    VA_HIP(h->ks.fwd(fa, bulk_grid(h, h->lanes_fwd), h->block, h->side));
-    VA_HIP(h->ks.rev(ra, bulk_grid(h, h->lanes_rev), h->block, h->side));
+    va::FwdArgs fb = fa;
+    va::RevArgs rb = ra;
+    if (im) {
+        if ((st = bulk_instance_major(h, p, &rb, 1, h->side))) return st;
+        fb.p = h->pim;
+    }
+    VA_HIP(h->ks.fwd(fb, bulk_grid(h, h->lanes_fwd), h->block, h->side));
+    VA_HIP(h->ks.rev(rb, bulk_grid(h, h->lanes_rev), h->block, h->side));
+    if (im && (st = pbar_to_mu(h, dJdp, 1, cost == 0, h->side, h->pbar_set))) return st;
     if ((st = join_side(h, s))) return st;
-    if (im && (st = pbar_to_mu(h, dJdp, 1, false, s))) return st;
     return order_by_length(h, 1, s);
 }
 
@@ -1075,6 +1100,7 @@ va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_
             // instance-major parameters and dJ/dp sums of the row kernels
             alloc((void **)&h->pim, sizeof(double) * (size_t)B * ks.np);
             alloc((void **)&h->pbar_im, sizeof(double) * (size_t)B * ks.np);
+            alloc((void **)&h->pbar_set, sizeof(int32_t) * (size_t)B);
             h->pbar_nout = 1;
         }
     } else {

@@ -210,32 +210,43 @@ __device__ __forceinline__ double growth_row(const double (&Arow)[NN], double r0
 // value is unchanged).
 __device__ __forceinline__ double growth_rate(double r) { return r + 0.0; }
 
-// The persistent row kernels read the parameters instance-major, [B][P] (the
-// instance's 8(N + N^2) bytes contiguous: the kernels take instances in a
+// Parameters and dJ/dp of the row kernels.  The persistent kernels over the
+// bulk of the batch (IM) read the parameters instance-major, [B][P] (the
+// instance's 8(N + N^2) bytes contiguous: they take instances in a
 // data-dependent order, and [P][B] reads would fetch a cache line per value),
 // and store each instance's dJ/dp sums instance-major as they are,
-// [nout][B][P]; the host forms 0.0 + sum (with a cost) or mu + sum (without
-// one, adjointSolve adds to mu) in [nout][P][B] afterwards.
-template <int NN, class Args>
+// [nout][B][P], marking the instance in pbar_set; the host then forms
+// 0.0 + sum (with a cost) or mu + sum (without one, adjointSolve adds to mu) in
+// [nout][P][B].  The long-instance kernels, a few hundred instances that must
+// not wait for those copies, read [P][B] and update mu themselves.
+template <int NN, bool IM, class Args>
 __device__ __forceinline__ double param(const Args &a, int64_t k, int64_t b)
 {
     constexpr int64_t P = NN + NN * NN;
-    return a.p[b * P + k];
+    if constexpr (IM) return a.p[b * P + k];
+    else return a.p[k * a.sp.B + b];
 }
 
-template <int NN>
+template <int NN, bool IM>
 __device__ __forceinline__ void put_pbar(const RevArgs &a, int64_t o, int64_t k, int64_t b,
                                          double x)
 {
     constexpr int64_t P = NN + NN * NN;
-    a.mu[(o * a.sp.B + b) * P + k] = x;
+    if constexpr (IM) {
+        a.mu[(o * a.sp.B + b) * P + k] = x;
+    } else {
+        double *m = a.mu + (o * P + k) * a.sp.B + b;
+        *m = a.cost ? 0.0 + x : *m + x;
+    }
 }
 
 // A failed instance's dJ/dp output: NaN.
-template <int NN>
+template <int NN, bool IM>
 __device__ __forceinline__ void put_pbar_nan(const RevArgs &a, int64_t o, int64_t k, int64_t b)
 {
-    put_pbar<NN>(a, o, k, b, __builtin_nan(""));
+    constexpr int64_t P = NN + NN * NN;
+    if constexpr (IM) a.mu[(o * a.sp.B + b) * P + k] = __builtin_nan("");
+    else a.mu[(o * P + k) * a.sp.B + b] = __builtin_nan("");
 }
 
 // xi_j = w_j g_j + sum_k u_k A_kj, split the same way.
@@ -354,9 +365,9 @@ __device__ __forceinline__ void glv_rows_fwd_body(const FwdArgs &a, int64_t one)
                 if (b >= q.hi) break;
                 if (a.order) b = a.order[b];
             }
-            r = growth_rate(param<NN>(a, i, b));
+            r = growth_rate(param<NN, !OneShot>(a, i, b));
 #pragma unroll
-            for (int j = 0; j < NN; ++j) Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
+            for (int j = 0; j < NN; ++j) Arow[j] = param<NN, !OneShot>(a, NN * (i + 1) + j, b);
             y = a.y[(int64_t)i * B + b];
             t = sp.t0;
             dt = sp.dt0;
@@ -571,9 +582,9 @@ __device__ __forceinline__ void glv_rows2_fwd_body(const FwdArgs &a)
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const int i = i0 + e;
-                r[e] = growth_rate(param<NN>(a, i, b));
+                r[e] = growth_rate(param<NN, true>(a, i, b));
 #pragma unroll
-                for (int j = 0; j < NN; ++j) A[e][j] = param<NN>(a, NN * (i + 1) + j, b);
+                for (int j = 0; j < NN; ++j) A[e][j] = param<NN, true>(a, NN * (i + 1) + j, b);
                 y[e] = a.y[(int64_t)i * B + b];
             }
             t = sp.t0;
@@ -805,17 +816,19 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
                 const int nseed = a.cost ? 1 : a.nout;
                 for (int oo = 0; oo < nseed; ++oo) {
                     a.lambda[((int64_t)oo * NN + i) * B + b] = qnan;
-                    put_pbar_nan<NN>(a, oo, i, b);
-                    for (int j = 0; j < NN; ++j) put_pbar_nan<NN>(a, oo, NN * (i + 1) + j, b);
+                    put_pbar_nan<NN, !OneShot>(a, oo, i, b);
+                    for (int j = 0; j < NN; ++j)
+                        put_pbar_nan<NN, !OneShot>(a, oo, NN * (i + 1) + j, b);
                 }
+                if (!OneShot && a.pbar_set && i == 0) a.pbar_set[b] = 1;
                 if (a.cost && i == 0) a.J[b] = qnan;
                 continue;
             }
-            r = growth_rate(param<NN>(a, i, b));
+            r = growth_rate(param<NN, !OneShot>(a, i, b));
 #pragma unroll
             for (int j = 0; j < NN; ++j) {
-                Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
-                Acol[j] = param<NN>(a, NN * (j + 1) + i, b);
+                Arow[j] = param<NN, !OneShot>(a, NN * (i + 1) + j, b);
+                Acol[j] = param<NN, !OneShot>(a, NN * (j + 1) + i, b);
             }
             rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
             o = 0;
@@ -933,9 +946,10 @@ __device__ __forceinline__ void glv_rows_rev_body(const RevArgs &a, int64_t one)
         if (!back) {
             const int64_t lo = (int64_t)o * NN;
             a.lambda[(lo + i) * B + b] = lam;
-            put_pbar<NN>(a, o, i, b, rbar);
+            put_pbar<NN, !OneShot>(a, o, i, b, rbar);
 #pragma unroll
-            for (int k = 0; k < NN; ++k) put_pbar<NN>(a, o, NN * (k + 1) + i, b, Abar[k]);
+            for (int k = 0; k < NN; ++k) put_pbar<NN, !OneShot>(a, o, NN * (k + 1) + i, b, Abar[k]);
+            if (!OneShot && a.pbar_set && i == 0) a.pbar_set[b] = 1;
             ++o;
             if (a.cost || o >= a.nout) {
                 need = true;
@@ -1020,10 +1034,10 @@ __device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, Stag
     const int64_t B = sp.B;
     const int32_t nacc = a.n_acc[b];
     if (a.status[b] != LANE_OK || nacc <= 0) return;
-    const double r = growth_rate(param<NN>(a, i, b));
+    const double r = growth_rate(param<NN, false>(a, i, b));
     double Arow[NN], v[NN];
 #pragma unroll
-    for (int j = 0; j < NN; ++j) Arow[j] = param<NN>(a, NN * (i + 1) + j, b);
+    for (int j = 0; j < NN; ++j) Arow[j] = param<NN, false>(a, NN * (i + 1) + j, b);
     const double *rows = a.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
     const TabRegs<Tab, true> tc;
     // (h_n, y_n,i) of the next two steps, loaded ahead of use
@@ -1077,8 +1091,8 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
     const double qnan = __builtin_nan("");
     if (a.status[b] != LANE_OK) {
         a.lambda[(int64_t)i * B + b] = qnan;
-        put_pbar_nan<NN>(a, 0, i, b);
-        for (int j = 0; j < NN; ++j) put_pbar_nan<NN>(a, 0, NN * (i + 1) + j, b);
+        put_pbar_nan<NN, false>(a, 0, i, b);
+        for (int j = 0; j < NN; ++j) put_pbar_nan<NN, false>(a, 0, NN * (i + 1) + j, b);
         if (a.cost) a.J[b] = qnan;
         return;
     }
@@ -1086,7 +1100,7 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
     double Acol[NN], Abar[NN], v[NN], vu[NN], rbar = 0.0, lam;
 #pragma unroll
     for (int j = 0; j < NN; ++j) {
-        Acol[j] = param<NN>(a, NN * (j + 1) + i, b);
+        Acol[j] = param<NN, false>(a, NN * (j + 1) + i, b);
         Abar[j] = 0.0;
     }
     if (a.cost) {
@@ -1153,9 +1167,9 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
         lam = ybar;
     }
     a.lambda[(int64_t)i * B + b] = lam;
-    put_pbar<NN>(a, 0, i, b, rbar);
+    put_pbar<NN, false>(a, 0, i, b, rbar);
 #pragma unroll
-    for (int kk = 0; kk < NN; ++kk) put_pbar<NN>(a, 0, NN * (kk + 1) + i, b, Abar[kk]);
+    for (int kk = 0; kk < NN; ++kk) put_pbar<NN, false>(a, 0, NN * (kk + 1) + i, b, Abar[kk]);
 }
 
 template <int NN, class Tab>

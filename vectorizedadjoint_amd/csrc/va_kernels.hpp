@@ -18,7 +18,7 @@ struct SolveParams {
 struct FwdArgs {
     SolveParams sp;
     double *y;             // [N][B] in y0, out y(T)
-    const double *p;       // [P][B]; Lotka-Volterra row kernels: [B][P]
+    const double *p;       // [P][B]; Lotka-Volterra persistent kernels: [B][P]
     double *traj;          // [B][cap+1][N+1] rows (t, y)
     int32_t *n_acc, *n_rej, *status;
     int32_t *counter;
@@ -32,8 +32,10 @@ struct FwdArgs {
 //   cost > 0: lambda computed from y(T) (read from yT) and written to dJdy0,
 //             dJdp = sum p_bar, J written.
 //   cost == 0: lambda [nout][N][B] in/out, mu [nout][P][B] += (reference semantics).
-// The Lotka-Volterra row kernels take p as [B][P] and store the dJ/dp sums
-// [nout][B][P] in mu as they are, for either cost (the caller applies them).
+// The Lotka-Volterra persistent kernels take p as [B][P], store the dJ/dp sums
+// [nout][B][P] in mu as they are, for either cost (the caller applies them),
+// and set pbar_set[b] for each instance they wrote (when pbar_set is given);
+// the long-instance kernels take p and mu as above.
 struct RevArgs {
     SolveParams sp;
     const double *p;
@@ -50,6 +52,7 @@ struct RevArgs {
     const int32_t *order;  // [B] or null: the k-th instance taken is order[k]
     const int32_t *split = nullptr;  // as in FwdArgs
     int32_t part = 0;
+    int32_t *pbar_set = nullptr;     // [B] or null
 };
 
 // Plain form: one launch per accepted step, state in HBM, no refill, full
