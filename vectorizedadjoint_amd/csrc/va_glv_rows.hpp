@@ -1424,15 +1424,23 @@ __global__ void __launch_bounds__(256) k_glv_rows_plain_rev(PlainRevArgs a)
 
 template <int NN, class Tab, bool Ad, bool WithLong = true>
 struct LaunchGlvRows {
-    // the persistent forward with two rows per lane where N is even and small
-    // enough for both rows' parameters to stay in registers
+    // The persistent forward with two rows per lane where N is even and small
+    // enough for both rows' parameters to stay in registers, when the instances
+    // come longest-first (a.order): a wave then holds instances of similar
+    // length.  In index order (a handle's first solve) a wave runs as long as the
+    // longest of its instances, and 12 unordered instances per wave made the
+    // first solve 18% slower than 6, so that forward keeps one row per lane.
     static constexpr bool kPairs = NN % 2 == 0 && NN <= 16;
     static hipError_t fwd(const FwdArgs &a, int grid, int block, hipStream_t s)
     {
-        if constexpr (kPairs)
-            hipLaunchKernelGGL((k_glv_rows2_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_glv_rows_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
+        if constexpr (kPairs) {
+            if (a.order) {
+                hipLaunchKernelGGL((k_glv_rows2_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s,
+                                   a);
+                return hipGetLastError();
+            }
+        }
+        hipLaunchKernelGGL((k_glv_rows_fwd<NN, Tab, Ad>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
     static hipError_t rev(const RevArgs &a, int grid, int block, hipStream_t s)
