@@ -104,57 +104,65 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
 // Instance-major copies for the Lotka-Volterra row kernels.  The persistent
 // kernels take instances in a data-dependent order, so reading [P][B]
 // parameters fetches a whole cache line for each 8-byte value; an instance's
-// P values read from one contiguous block of 8P bytes instead.  A block
-// transposes a tile of 64 instances through LDS (row stride 65 against bank
-// conflicts) with coalesced reads and writes on both sides.
-constexpr int VA_TILE_B = 64;  // instances per tile
-constexpr int VA_TILE_K = 96;  // parameters per tile: 96 x 65 doubles of LDS
+// P values read from one contiguous block of 8P bytes instead.  A block of 256
+// threads transposes 32 instances through LDS, 128 parameters per pass: on the
+// [P][B] side each wave moves two 256-byte runs of 32 instances per
+// instruction, on the [B][P] side each wave moves 64 consecutive parameters of
+// one instance (an instance's parameters, and the block's 32 instances, are
+// contiguous there).  The row stride of 129 doubles keeps the 32 instances of a
+// parameter in distinct banks.
+constexpr int VA_TILE_B = 32;   // instances per block
+constexpr int VA_TILE_K = 128;  // parameters per pass: 32 x 129 doubles of LDS
 
 __global__ void __launch_bounds__(256) k_to_instance_major(const double *src, double *dst,
                                                            int64_t P, int64_t B)
 {
-    __shared__ double tile[VA_TILE_K * 65];
-    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B, k0 = (int64_t)blockIdx.y * VA_TILE_K;
+    __shared__ double tile[VA_TILE_B][VA_TILE_K + 1];
+    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B;
     const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
-    const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
-    for (int idx = threadIdx.x; idx < kn * VA_TILE_B; idx += blockDim.x) {
-        const int k = idx / VA_TILE_B, bb = idx % VA_TILE_B;
-        if (bb < nb) tile[k * 65 + bb] = src[(k0 + k) * B + b0 + bb];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * kn; idx += blockDim.x) {
-        const int bb = idx / kn, k = idx % kn;
-        dst[(b0 + bb) * P + k0 + k] = tile[k * 65 + bb];
+    const int bb = threadIdx.x & 31, kr = threadIdx.x >> 5;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t k0 = 0; k0 < P; k0 += VA_TILE_K) {
+        const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
+        if (bb < nb)
+            for (int k = kr; k < kn; k += 8) tile[bb][k] = src[(k0 + k) * B + b0 + bb];
+        __syncthreads();
+        for (int c = wv; c < nb; c += 4)
+            for (int k = lane; k < kn; k += 64) dst[(b0 + c) * P + k0 + k] = tile[c][k];
+        __syncthreads();
     }
 }
 
 // dJ/dp sums of the row kernels, instance-major [nout][B][P], into mu
 // [nout][P][B]: 0.0 + sum with a cost, mu + sum without one (the updates the
 // kernels make when they write mu themselves); with set, only the instances
-// marked there (the others' mu is the long-instance kernel's).  blockIdx.z is
+// marked there (the others' mu is the long-instance kernel's).  blockIdx.y is
 // the seed.
 __global__ void __launch_bounds__(256) k_from_instance_major(const double *src, double *dst,
                                                              int64_t P, int64_t B, int add,
                                                              const int32_t *set)
 {
-    __shared__ double tile[VA_TILE_K * 65];
-    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B, k0 = (int64_t)blockIdx.y * VA_TILE_K;
+    __shared__ double tile[VA_TILE_B][VA_TILE_K + 1];
+    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B;
     const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
-    const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
-    const double *sp = src + (int64_t)blockIdx.z * B * P;
-    double *dp = dst + (int64_t)blockIdx.z * P * B;
-    for (int idx = threadIdx.x; idx < nb * kn; idx += blockDim.x) {
-        const int bb = idx / kn, k = idx % kn;
-        tile[k * 65 + bb] = sp[(b0 + bb) * P + k0 + k];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < kn * VA_TILE_B; idx += blockDim.x) {
-        const int k = idx / VA_TILE_B, bb = idx % VA_TILE_B;
-        if (bb < nb && (!set || set[b0 + bb])) {
-            double *m = dp + (k0 + k) * B + b0 + bb;
-            const double x = tile[k * 65 + bb];
-            *m = add ? *m + x : 0.0 + x;
+    const int bb = threadIdx.x & 31, kr = threadIdx.x >> 5;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double *sp = src + (int64_t)blockIdx.y * B * P;
+    double *dp = dst + (int64_t)blockIdx.y * P * B;
+    const bool mine = bb < nb && (!set || set[b0 + bb]);
+    for (int64_t k0 = 0; k0 < P; k0 += VA_TILE_K) {
+        const int kn = (int)((P - k0 < VA_TILE_K) ? P - k0 : VA_TILE_K);
+        for (int c = wv; c < nb; c += 4)
+            for (int k = lane; k < kn; k += 64) tile[c][k] = sp[(b0 + c) * P + k0 + k];
+        __syncthreads();
+        if (mine) {
+            for (int k = kr; k < kn; k += 8) {
+                double *m = dp + (k0 + k) * B + b0 + bb;
+                const double x = tile[bb][k];
+                *m = add ? *m + x : 0.0 + x;
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -325,6 +333,9 @@ struct va_handle {
     int32_t graph_cost = 0;
     int32_t *host_counter = nullptr;
     bool have_forward = false;
+    // order_fwd holds the longest-first order of an earlier forward (and
+    // counters[5] its count of long instances); until then forwards take index order
+    bool ordered = false;
     // va_solve with the long instances on a second stream (fork / join events)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -429,6 +440,7 @@ va_status order_by_length(va_handle *h, int mode, hipStream_t s)
     if (st) return st;
     const int gb = blocks_for(B, 256);
     int32_t *out = mode == 0 ? h->order : h->order_fwd;
+    if (mode == 1) h->ordered = true;
     hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej, h->status,
                        B, mode, h->order_tmp);
     hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(VA_ORDER_BUCKETS), 0, s, h->order_tmp,
@@ -441,15 +453,23 @@ va_status order_by_length(va_handle *h, int mode, hipStream_t s)
 
 va_status order_longest_first(va_handle *h, hipStream_t s) { return order_by_length(h, 0, s); }
 
-// The forward order is used unless the caller asked for index order: where
-// instances are refilled (more instances than are in flight), so that the long
-// ones start first, and with one lane per instance also when all are resident,
-// because the 64 instances that share a wave then have similar lengths (a wave
-// runs as long as its longest instance).
-bool reuse_order(const va_handle *h, int64_t lanes)
+// Forwards take instances longest-first by an earlier forward's attempts unless
+// the caller asked for index order: where instances are refilled (more
+// instances than are in flight), so that the long ones start first, and with one
+// lane per instance also when all are resident, because the 64 instances that
+// share a wave then have similar lengths (a wave runs as long as its longest
+// instance).
+bool order_wanted(const va_handle *h, int64_t lanes)
 {
     if (h->pb.flags & VA_FLAG_COLD_ORDER) return false;
     return h->pb.batch > lanes || (h->ks.layout == 0 && h->pb.batch > 64);
+}
+
+// The order is wanted and an earlier forward has left it (a handle's first
+// forward runs in index order).
+bool reuse_order(const va_handle *h, int64_t lanes)
+{
+    return h->ordered && order_wanted(h, lanes);
 }
 
 // ---------------- persistent form ----------------
@@ -505,8 +525,8 @@ bool instance_major(const va_handle *h) { return h->ks.layout == 1 && h->pim != 
 
 va_status to_instance_major(const double *p, double *pim, int64_t P, int64_t B, hipStream_t s)
 {
-    const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)((P + VA_TILE_K - 1) / VA_TILE_K));
-    hipLaunchKernelGGL(k_to_instance_major, grid, dim3(256), 0, s, p, pim, P, B);
+    hipLaunchKernelGGL(k_to_instance_major, dim3(blocks_for(B, VA_TILE_B)), dim3(256), 0, s, p,
+                       pim, P, B);
     VA_HIP(hipGetLastError());
     return VA_OK;
 }
@@ -516,8 +536,7 @@ va_status to_instance_major(const double *p, double *pim, int64_t P, int64_t B, 
 va_status from_instance_major(const double *pbar_im, double *mu, int64_t P, int64_t B, int nout,
                               bool add, hipStream_t s, const int32_t *set = nullptr)
 {
-    const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)((P + VA_TILE_K - 1) / VA_TILE_K),
-                    (unsigned)nout);
+    const dim3 grid(blocks_for(B, VA_TILE_B), (unsigned)nout);
     hipLaunchKernelGGL(k_from_instance_major, grid, dim3(256), 0, s, pbar_im, mu, P, B,
                        add ? 1 : 0, set);
     VA_HIP(hipGetLastError());
@@ -603,7 +622,7 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
         }
         VA_HIP(h->ks.fwd(a, blocks_for(lanes, h->ks.slots(h->block)), h->block, s));
     }
-    if (reuse) return order_by_length(h, 1, s);
+    if (order_wanted(h, h->lanes_fwd)) return order_by_length(h, 1, s);
     return VA_OK;
 }
 
@@ -1263,7 +1282,10 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
                                 dJdp, n_accepted, n_rejected, lane_status, stream);
     }
     if ((st = check_rows(h))) return st;
-    if (!(h->pb.flags & VA_FLAG_GRAPH) || h->pb.mode != VA_MODE_PERSISTENT)
+    // A handle's first solve (index order, no long instances known yet) runs
+    // unrecorded; the graph records the solve that reuses the order.
+    if (!(h->pb.flags & VA_FLAG_GRAPH) || h->pb.mode != VA_MODE_PERSISTENT ||
+        (!h->ordered && order_wanted(h, h->lanes_fwd)))
         return solve_enqueue(h, y0, p, cost, cost_weights, J, yT, dJdy0, dJdp, n_accepted,
                              n_rejected, lane_status, stream);
     const void *key[12] = {y0, p, cost_weights, J, yT, dJdy0, dJdp, n_accepted, n_rejected,
@@ -1390,7 +1412,7 @@ static va_status observed_enqueue(va_handle *h, const double *y0, const double *
     VA_HIP(h->ks.obs_fwd(f, blocks_for(std::min<int64_t>(h->lanes_obs_fwd, B), h->block),
                          h->block, s));
     h->have_forward = !h->obs_ckpt;
-    if (reuse && (st = order_by_length(h, 1, s))) return st;
+    if (order_wanted(h, h->lanes_obs_fwd) && (st = order_by_length(h, 1, s))) return st;
     if ((st = order_longest_first(h, s))) return st;
     va::ObsRevArgs r;
     r.sp = f.sp;

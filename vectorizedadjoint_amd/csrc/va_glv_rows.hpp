@@ -1005,20 +1005,33 @@ struct StageRing {
     int abort;                              // a wait gave up: both waves stop
 };
 
-__device__ __forceinline__ int lds_read_volatile(const int *p) { return *(const volatile int *)p; }
-__device__ __forceinline__ void lds_write_volatile(int *p, int v) { *(volatile int *)p = v; }
+// The ring's counters are relaxed workgroup-scope atomics (single LDS reads and
+// writes, no wait for their completion); the fences around them order the
+// slot accesses.
+__device__ __forceinline__ int ring_load(int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void ring_store(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Wait until *ctr > k.  Every wait ends: it returns false when the other wave
 // gave up, and gives up itself (setting abort) after VA_SPIN_LIMIT polls.
 template <class Ring>
-__device__ __forceinline__ bool wait_above(Ring &R, const int *ctr, int k)
+__device__ __forceinline__ bool wait_above(Ring &R, int *ctr, int k)
 {
     for (int spins = 0; spins < VA_SPIN_LIMIT; ++spins) {
-        if (lds_read_volatile(ctr) > k) return true;
-        if (lds_read_volatile(&R.abort)) return false;
+        if (ring_load(ctr) > k) {
+            // the other wave's slot accesses before its count come first
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            return true;
+        }
+        if (ring_load(&R.abort)) return false;
         __builtin_amdgcn_s_sleep(1);
     }
-    lds_write_volatile(&R.abort, 1);
+    ring_store(&R.abort, 1);
     return false;
 }
 
@@ -1073,7 +1086,7 @@ __device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, Stag
         R.h[slot] = h;
         // the slot's writes complete before the count that publishes them
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        lds_write_volatile(&R.avail, k + 1);
+        ring_store(&R.avail, k + 1);
     }
 }
 
@@ -1139,8 +1152,8 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
         }
         const double h = R.h[slot];
         // the slot's values are in registers: release it to wave 1
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        lds_write_volatile(&R.consumed, k + 1);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        ring_store(&R.consumed, k + 1);
         double Kb[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) Kb[m] = (tc.b[m] * h) * lam;
