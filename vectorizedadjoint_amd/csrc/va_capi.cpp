@@ -1110,8 +1110,9 @@ va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_
     if (ks.layout == 2) {
         alloc((void **)&h->abar_part, sizeof(double) * (size_t)h->shared_grid * ks.n * ks.n);
         // per workgroup: waves x s stages x 64 lanes x 4 growth values
+        // the reverse's growth products and stage inputs, per wave and stage
         alloc((void **)&h->shared_gscr, sizeof(double) * (size_t)h->shared_grid *
-                                            ks.shared_waves * ks.s_rev * 64 * 4);
+                                            ks.shared_waves * 2 * ks.s_rev * 64 * 4);
     }
     if (pb->mode == VA_MODE_PERSISTENT) {
         if (full_store) alloc((void **)&h->traj, sizeof(double) * rows * (size_t)B * (ks.n + 2));

@@ -50,11 +50,13 @@ __device__ __forceinline__ d4 mfma(double a, double b, d4 c)
 
 // G[inst][16w + c] = sum_k X[inst][k] A[16w + c][k] for the tile (X staged in Xs).
 // The forward and the reverse's recompute call this same sequence, so they
-// produce the same bits.
+// produce the same bits (U, the unroll factor, only sets how far operand loads
+// run ahead: the reverse, short of registers, uses 4).
+template <int U = 8>
 __device__ __forceinline__ d4 gemm_xat(const double *Xs, const double *As, int w, int r, int c)
 {
     d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
+#pragma unroll U
     for (int kb = 0; kb < GS_N / 4; ++kb) {
         const int k = 4 * kb + r;
         acc = mfma(Xs[sw(c, k)], As[sw(16 * w + c, k)], acc);
@@ -66,7 +68,7 @@ __device__ __forceinline__ d4 gemm_xat(const double *Xs, const double *As, int w
 __device__ __forceinline__ d4 gemm_ua(const double *Us, const double *As, int w, int r, int c)
 {
     d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
+#pragma unroll 2
     for (int kb = 0; kb < GS_N / 4; ++kb) {
         const int k = 4 * kb + r;
         acc = mfma(Us[sw(c, k)], As[sw(k, 16 * w + c)], acc);
@@ -78,7 +80,9 @@ __device__ __forceinline__ d4 gemm_ua(const double *Us, const double *As, int w,
 __device__ __forceinline__ void gemm_utx(const double *Us, const double *Xs, int w, int r, int c,
                                          d4 (&abar)[GS_N / 16])
 {
-#pragma unroll
+    // one instance row at a time: the 8 accumulators are live across the
+    // call, so operands are not loaded further ahead
+#pragma unroll 1
     for (int kb = 0; kb < GS_T / 4; ++kb) {
         const int inst = 4 * kb + r;
         const double a = Us[sw(inst, 16 * w + c)];
@@ -317,8 +321,11 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
 #pragma unroll
     for (int jb = 0; jb < GS_N / 16; ++jb) abar[jb] = d4{0.0, 0.0, 0.0, 0.0};
     const int64_t ntiles = (B + GS_T - 1) / GS_T;
-    // growth products of the S stages of the step being reversed, [S][64 lanes] per wave
-    d4 *gw = reinterpret_cast<d4 *>(a.gscr) + ((size_t)blockIdx.x * (GS_THREADS / 64) + w) * S * 64;
+    // growth products and stage inputs of the S stages of the step being reversed,
+    // [2][S][64 lanes] per wave: the backward reads them back instead of keeping
+    // the stage values live in registers, which the dJ/dA accumulators fill
+    d4 *gw = reinterpret_cast<d4 *>(a.gscr) + ((size_t)blockIdx.x * (GS_THREADS / 64) + w) * 2 * S * 64;
+    d4 *xw = gw + S * 64;
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         int64_t b[4];
@@ -358,9 +365,9 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                     yn[jj] = row[2 + comp];
                 }
             }
-            // recompute the stages from y_n (backpropagation.hpp:24-64); K_m is kept in
-            // registers (X_m is rebuilt from it), the growth products A X_m go to this
-            // wave's scratch rows, where the adjoint reads them back
+            // recompute the stages from y_n (backpropagation.hpp:24-64); the stage
+            // inputs X_m and growth products A X_m go to this wave's scratch rows,
+            // where the adjoint reads them back
             for (int m = 0; m < S; ++m) {
                 double X[4];
 #pragma unroll
@@ -369,9 +376,10 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                     Xs[sw(r + 4 * jj, comp)] = X[jj];
                 }
                 __syncthreads();
-                const d4 G = gemm_xat(Xs, As, w, r, c);
+                const d4 G = gemm_xat<4>(Xs, As, w, r, c);
                 __syncthreads();
                 gw[m * 64 + lane] = G;
+                xw[m * 64 + lane] = d4{X[0], X[1], X[2], X[3]};
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) K[m][jj] = X[jj] * (rr[jj] + G[jj]);
             }
@@ -383,11 +391,12 @@ __global__ void __launch_bounds__(GS_THREADS) k_glvs_rev(SharedRevArgs a)
                 for (int m = 0; m < S; ++m) Kb[m][jj] = (Tab::B[m] * h[jj]) * lam[jj];
             }
             for (int m = S - 1; m >= 0; --m) {
-                const d4 G = gw[m * 64 + lane];  // this lane's own growth products of stage m
+                const d4 G = gw[m * 64 + lane];   // this lane's own growth products of stage m
+                const d4 Xm = xw[m * 64 + lane];  // and its stage inputs
                 double u[4];
 #pragma unroll
                 for (int jj = 0; jj < 4; ++jj) {
-                    const double X = stage_x<Tab>(m, h[jj], yn[jj], K, jj);
+                    const double X = Xm[jj];
                     u[jj] = on[jj] ? Kb[m][jj] * X : 0.0;
                     Us[sw(r + 4 * jj, comp)] = u[jj];
                     Xs[sw(r + 4 * jj, comp)] = X;

@@ -161,7 +161,7 @@ struct SharedRevArgs {
     double *dJdy0;         // [N][B]
     double *dJdr;          // [N][B]
     double *abar_part;     // [grid][N][N] per-workgroup partial dJ/dA
-    double *gscr;          // [grid][waves][S][64][4] growth products of the step being reversed
+    double *gscr;          // [grid][waves][2][S][64][4] growth products and stage inputs of the step being reversed
 };
 
 // Layout version of KernelSet: kernel tables made in a caller's translation unit
