@@ -713,8 +713,8 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.part = 1;
     const bool im = instance_major(h);
     if ((st = fork_side(h, s))) return st;
-    // one block of two waves per long instance
-    VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 128, s));
+    // one block of three waves per long instance
+    VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 192, s));
     va::FwdArgs fb = fa;
     va::RevArgs rb = ra;
     if (im) {

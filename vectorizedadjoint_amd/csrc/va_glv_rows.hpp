@@ -977,16 +977,20 @@ __global__ void __launch_bounds__(256) k_glv_rows_rev(RevArgs a)
 
 // ---------------------------------------------------------------------------
 // Instances expected to run long: queue positions [0, *split) of the forward
-// order, one block of two waves per instance.  Wave 0 runs the instance's
+// order, one block of three waves per instance.  Wave 0 runs the instance's
 // forward and then at once the backward of its reverse, so it never waits for
 // another instance; the rest of the batch (positions from *split) runs in the
 // persistent kernels on another stream meanwhile.  During the reverse, wave 1
 // forms the stage values of each stored step -- the stage inputs X_m and the
 // growth rates g_m from (h_n, y_n) with h_n = t_{n+1} - t_n, the operations of
 // the recompute in glv_rows_rev_body -- a few steps ahead of wave 0 into a ring
-// of LDS slots, so wave 0's dependent chain holds the backward alone
-// (backpropagation.hpp:104-153).  The two waves meet through two counters in
-// LDS: the steps staged and the steps whose slot wave 0 has read.
+// of LDS slots, and wave 2 accumulates dJ/dr and dJ/dA from the adjoint
+// products u that wave 0 leaves in the same slot, so wave 0's dependent chain
+// holds the adjoint recursion alone (backpropagation.hpp:104-153).  The waves
+// meet through three counters in LDS: the steps staged (wave 1), the steps
+// whose products are written (wave 0) and the steps accumulated (wave 2, which
+// frees the slot; it cannot pass wave 0, so wave 0 needs no count of its own
+// to reuse a slot).
 // ---------------------------------------------------------------------------
 constexpr int VA_STAGE_RING = 8;         // steps staged ahead of the backward
 constexpr int VA_SPIN_LIMIT = 1 << 22;   // polls of a ring counter (about 0.1 s) before giving up
@@ -999,10 +1003,12 @@ constexpr int VA_LONG_LDS_PAD = 68 * 1024;
 template <int NN, class Tab>
 struct StageRing {
     double2 xg[VA_STAGE_RING][Tab::S][NN];  // (X_m,i, g_m,i) of a staged step
+    double u[VA_STAGE_RING][Tab::S][NN];    // its adjoint products u_m,i = w_m,i X_m,i
     double h[VA_STAGE_RING];                // its h_n
     int avail;                              // steps staged (wave 1)
-    int consumed;                           // steps whose slot wave 0 has read
-    int abort;                              // a wait gave up: both waves stop
+    int uavail;                             // steps whose products wave 0 has written
+    int done;                               // steps wave 2 has accumulated: slots free
+    int abort;                              // a wait gave up: every wave stops
 };
 
 // The ring's counters are relaxed workgroup-scope atomics (single LDS reads and
@@ -1068,8 +1074,9 @@ __device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, Stag
             h1 = rows[(size_t)(n - 2) * F + 1];
             y1 = rows[(size_t)(n - 2) * F + 2 + i];
         }
-        // the slot of step k - RING must have been read by wave 0
-        if (k >= VA_STAGE_RING && !wait_above(R, &R.consumed, k - VA_STAGE_RING)) return;
+        // the slot of step k - RING must have been accumulated by wave 2 (and so
+        // read by wave 0 before that)
+        if (k >= VA_STAGE_RING && !wait_above(R, &R.done, k - VA_STAGE_RING)) return;
         const int slot = k % VA_STAGE_RING;
         double Kn[S];
 #pragma unroll
@@ -1090,8 +1097,9 @@ __device__ __forceinline__ void glv_long_stage(const RevArgs &a, int64_t b, Stag
     }
 }
 
-// Wave 0 after the forward: terminal cost, then the backward of steps
-// n_acc-1 .. 0 from the staged values (the backward of glv_rows_rev_body).
+// Wave 0 after the forward: terminal cost, then the adjoint recursion of steps
+// n_acc-1 .. 0 from the staged values (the backward of glv_rows_rev_body),
+// leaving each stage's products u in the step's slot for wave 2.
 template <int NN, class Tab>
 __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
                                                   StageRing<NN, Tab> &R)
@@ -1110,12 +1118,9 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
         return;
     }
     const TabRegs<Tab, true> tc;
-    double Acol[NN], Abar[NN], v[NN], vu[NN], rbar = 0.0, lam;
+    double Acol[NN], v[NN], vu[NN], lam;
 #pragma unroll
-    for (int j = 0; j < NN; ++j) {
-        Acol[j] = param<NN, false>(a, NN * (j + 1) + i, b);
-        Abar[j] = 0.0;
-    }
+    for (int j = 0; j < NN; ++j) Acol[j] = param<NN, false>(a, NN * (j + 1) + i, b);
     if (a.cost) {
         const double yT = a.yT[(int64_t)i * B + b];
         // terminal cost: J = fold over i in order, lambda_i from y_i(T)
@@ -1137,9 +1142,7 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
     const int32_t nacc = a.n_acc[b];
     for (int k = 0; k < nacc; ++k) {
         if (!wait_above(R, &R.avail, k)) {
-            lam = rbar = qnan;
-#pragma unroll
-            for (int j = 0; j < NN; ++j) Abar[j] = qnan;
+            lam = qnan;
             break;
         }
         const int slot = k % VA_STAGE_RING;
@@ -1151,9 +1154,6 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
             G[m] = xg.y;
         }
         const double h = R.h[slot];
-        // the slot's values are in registers: release it to wave 1
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        ring_store(&R.consumed, k + 1);
         double Kb[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) Kb[m] = (tc.b[m] * h) * lam;
@@ -1165,12 +1165,9 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
             const double xown = X[m];
             const double u = w * xown;
             const double wg = w * G[m];
+            R.u[slot][m][i] = u;
             bcast_group0<NN>(u, vu);
             const double xi = adjoint_col<NN>(Acol, wg, vu);
-            rbar = rbar + u;
-            // column i of dJ/dA: dA_ki += u_k X_i, each product rounded before it is added
-#pragma unroll
-            for (int kk = 0; kk < NN; ++kk) Abar[kk] = Abar[kk] + vu[kk] * xown;
             ybar = ybar + xi;
             // Kbar_j += (xi a_mj) h (detail/backpropagation.hpp:215)
 #pragma unroll
@@ -1178,15 +1175,62 @@ __device__ __forceinline__ void glv_long_backward(const RevArgs &a, int64_t b,
                 if (j < m && Tab::A[m][j] != 0.0) Kb[j] = Kb[j] + (xi * tc.a[m][j]) * h;
         }
         lam = ybar;
+        // the step's products are written before the count that publishes them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        ring_store(&R.uavail, k + 1);
     }
     a.lambda[(int64_t)i * B + b] = lam;
+}
+
+// Wave 2: dJ/dr_i and column i of dJ/dA from the products u of each step wave 0
+// has reversed, in wave 0's stage order (what glv_rows_rev_body accumulates
+// beside its recursion, the same operations in the same order).
+template <int NN, class Tab>
+__device__ __forceinline__ void glv_long_accumulate(const RevArgs &a, int64_t b,
+                                                    StageRing<NN, Tab> &R)
+{
+    constexpr int S = Tab::S;
+    const int i = threadIdx.x & 63;
+    if (i >= NN) return;
+    __builtin_amdgcn_s_setprio(3);
+    // wave 0 writes the outputs of an instance that did not end OK
+    if (a.status[b] != LANE_OK) return;
+    const double qnan = __builtin_nan("");
+    double Abar[NN], vu[NN], rbar = 0.0;
+#pragma unroll
+    for (int j = 0; j < NN; ++j) Abar[j] = 0.0;
+    const int32_t nacc = a.n_acc[b];
+    for (int k = 0; k < nacc; ++k) {
+        if (!wait_above(R, &R.uavail, k)) {
+            rbar = qnan;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) Abar[j] = qnan;
+            break;
+        }
+        const int slot = k % VA_STAGE_RING;
+#pragma unroll
+        for (int ii = 0; ii < S; ++ii) {
+            const int m = S - 1 - ii;
+#pragma unroll
+            for (int kk = 0; kk < NN; ++kk) vu[kk] = R.u[slot][m][kk];
+            const double xown = R.xg[slot][m][i].x;
+            const double u = R.u[slot][m][i];
+            rbar = rbar + u;
+            // column i of dJ/dA: dA_ki += u_k X_i, each product rounded before it is added
+#pragma unroll
+            for (int kk = 0; kk < NN; ++kk) Abar[kk] = Abar[kk] + vu[kk] * xown;
+        }
+        // the slot's values are read: release it to waves 0 and 1
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        ring_store(&R.done, k + 1);
+    }
     put_pbar<NN, false>(a, 0, i, b, rbar);
 #pragma unroll
     for (int kk = 0; kk < NN; ++kk) put_pbar<NN, false>(a, 0, NN * (kk + 1) + i, b, Abar[kk]);
 }
 
 template <int NN, class Tab>
-__global__ void __launch_bounds__(128) k_glv_rows_long(FwdArgs fa, RevArgs ra)
+__global__ void __launch_bounds__(192) k_glv_rows_long(FwdArgs fa, RevArgs ra)
 {
     __shared__ StageRing<NN, Tab> ring;
     __shared__ double lds_pad[VA_LONG_LDS_PAD / 8];
@@ -1197,15 +1241,17 @@ __global__ void __launch_bounds__(128) k_glv_rows_long(FwdArgs fa, RevArgs ra)
     const int64_t b = fa.order[w];
     if (threadIdx.x == 0) {
         ring.avail = 0;
-        ring.consumed = 0;
+        ring.uavail = 0;
+        ring.done = 0;
         ring.abort = 0;
     }
     if (threadIdx.x < 64) glv_rows_fwd_body<NN, Tab, true, true>(fa, b);
-    // the forward's rows, y(T), counts and status become visible to both waves
+    // the forward's rows, y(T), counts and status become visible to every wave
     // (the barrier orders the block's memory operations at workgroup scope)
     __syncthreads();
     if (threadIdx.x < 64) glv_long_backward<NN, Tab>(ra, b, ring);
-    else glv_long_stage<NN, Tab>(ra, b, ring);
+    else if (threadIdx.x < 128) glv_long_stage<NN, Tab>(ra, b, ring);
+    else glv_long_accumulate<NN, Tab>(ra, b, ring);
 }
 
 // The same instances in a forward alone (va_forward) and in a reverse alone

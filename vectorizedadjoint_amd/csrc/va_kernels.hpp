@@ -175,8 +175,8 @@ struct KernelSet {
     int flop_rhs, flop_vjp;
     hipError_t (*fwd)(const FwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*rev)(const RevArgs &, int grid, int block, hipStream_t);
-    // forward then reverse of each long-running instance, one block of two waves per
-    // instance: grid = instances, block = 128 (null where absent)
+    // forward then reverse of each long-running instance, one block of three waves per
+    // instance: grid = instances, block = 192 (null where absent)
     hipError_t (*long_fused)(const FwdArgs &, const RevArgs &, int grid, int block, hipStream_t);
     // the same instances' forward alone and reverse alone (null where absent)
     hipError_t (*long_fwd)(const FwdArgs &, int grid, int block, hipStream_t);
