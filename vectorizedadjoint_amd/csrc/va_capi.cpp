@@ -305,11 +305,16 @@ struct va_handle {
     int32_t *pl_done = nullptr;
     // bookkeeping [B] and counters
     int32_t *n_acc = nullptr, *n_rej = nullptr, *status = nullptr;
-    // [0] fwd queue, [1] rev queue, [2] max steps, [3] active, [5] long instances
-    // at the head of order_fwd (from the previous forward's attempts)
+    // [0] fwd queue, [1] rev queue, [2] max steps, [3] active, [nlong_idx] long
+    // instances at the head of order_fwd (from the previous forward's attempts),
+    // [11 - nlong_idx] those of order_fwd_next
     int32_t *counters = nullptr;
     int32_t *order = nullptr;     // [B] reverse order, longest first
     int32_t *order_fwd = nullptr; // [B] next forward's order (previous attempts, longest first)
+    // [B] the order a split solve builds for the solve after it while its own
+    // reverse still reads order_fwd; the two are swapped when it returns
+    int32_t *order_fwd_next = nullptr;
+    int nlong_idx = 5;
     int32_t *order_tmp = nullptr; // [2 * VA_ORDER_BUCKETS] histogram, cursors
     // observed solves
     int64_t lanes_obs_fwd = 0, lanes_obs_rev = 0;
@@ -336,9 +341,11 @@ struct va_handle {
     // order_fwd holds the longest-first order of an earlier forward (and
     // counters[5] its count of long instances); until then forwards take index order
     bool ordered = false;
-    // va_solve with the long instances on a second stream (fork / join events)
+    // va_solve with the long instances on a second stream (fork / join events,
+    // and the end of the bulk's forward)
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fwd = nullptr;
+    bool capturing = false;  // solve_enqueue is being recorded into a graph
 };
 
 namespace {
@@ -412,6 +419,7 @@ void release(va_handle *h)
     hipFree(h->counters);
     hipFree(h->order);
     hipFree(h->order_fwd);
+    hipFree(h->order_fwd_next);
     hipFree(h->order_tmp);
     hipFree(h->obs_ck);
     hipFree(h->obs_seg);
@@ -428,25 +436,31 @@ void release(va_handle *h)
     if (h->side) hipStreamDestroy(h->side);
     if (h->ev_fork) hipEventDestroy(h->ev_fork);
     if (h->ev_join) hipEventDestroy(h->ev_join);
+    if (h->ev_fwd) hipEventDestroy(h->ev_fwd);
     if (h->host_counter) hipHostFree(h->host_counter);
 }
 
 // Longest-first order over the instances of the last forward: mode 0 into
-// h->order (the reverse), mode 1 into h->order_fwd (the next forward).
+// h->order (the reverse, by accepted steps), mode 1 into h->order_fwd (the next
+// forward, by attempts, with its count of long instances), mode 2 as mode 1 into
+// h->order_fwd_next.
 va_status order_by_length(va_handle *h, int mode, hipStream_t s)
 {
     const int64_t B = h->pb.batch;
     va_status st = fill_i32(h->order_tmp, 0, 2 * VA_ORDER_BUCKETS, s);
     if (st) return st;
     const int gb = blocks_for(B, 256);
-    int32_t *out = mode == 0 ? h->order : h->order_fwd;
-    if (mode == 1) h->ordered = true;
+    int32_t *out = mode == 0 ? h->order : mode == 1 ? h->order_fwd : h->order_fwd_next;
+    int32_t *n_long = mode == 0 ? nullptr
+                      : h->counters + (mode == 1 ? h->nlong_idx : 11 - h->nlong_idx);
+    const int by = mode == 0 ? 0 : 1;
+    if (mode != 0) h->ordered = true;
     hipLaunchKernelGGL(k_order_hist, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej, h->status,
-                       B, mode, h->order_tmp);
+                       B, by, h->order_tmp);
     hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(VA_ORDER_BUCKETS), 0, s, h->order_tmp,
-                       h->order_tmp + VA_ORDER_BUCKETS, mode == 1 ? h->counters + 5 : nullptr);
+                       h->order_tmp + VA_ORDER_BUCKETS, n_long);
     hipLaunchKernelGGL(k_order_scatter, dim3(gb), dim3(256), 0, s, h->n_acc, h->n_rej,
-                       h->status, B, mode, h->order_tmp + VA_ORDER_BUCKETS, out);
+                       h->status, B, by, h->order_tmp + VA_ORDER_BUCKETS, out);
     VA_HIP(hipGetLastError());
     return VA_OK;
 }
@@ -604,7 +618,7 @@ va_status forward_persistent(va_handle *h, double *y, const double *p, hipStream
     a.order = reuse ? h->order_fwd : nullptr;
     const int64_t lanes = std::min<int64_t>(h->lanes_fwd, B);
     if (split_solve(h) && h->ks.long_fwd) {
-        a.split = h->counters + 5;
+        a.split = h->counters + h->nlong_idx;
         a.part = 1;
         if ((st = fork_side(h, s))) return st;
         VA_HIP(h->ks.long_fwd(a, VA_LONG_MAX / 4, 256, s));
@@ -655,7 +669,7 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     const bool im = instance_major(h);
     const int nseed = cost ? 1 : nout;
     if (split) {
-        a.split = h->counters + 5;
+        a.split = h->counters + h->nlong_idx;
         a.part = 1;
         if ((st = fork_side(h, s))) return st;
         VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
@@ -676,8 +690,12 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
     return VA_OK;
 }
 
+va_status copy_out(va_handle *h, int32_t *n_acc, int32_t *n_rej, int32_t *lane_status,
+                   hipStream_t s);
+
 va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const double *cost_w,
-                      double *J, double *dJdy0, double *dJdp, hipStream_t s)
+                      double *J, double *dJdy0, double *dJdp, int32_t *n_accepted,
+                      int32_t *n_rejected, int32_t *lane_status, hipStream_t s)
 {
     va_status st = fill_i32(h->counters, 0, 4, s);
     if (st) return st;
@@ -692,7 +710,7 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     fa.counter = h->counters + 0;
     fa.max_steps_seen = h->counters + 2;
     fa.order = h->order_fwd;
-    fa.split = h->counters + 5;
+    fa.split = h->counters + h->nlong_idx;
     fa.part = 1;
     va::RevArgs ra;
     ra.sp = fa.sp;
@@ -709,23 +727,44 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.nout = 1;
     ra.counter = h->counters + 1;
     ra.order = h->order_fwd;
-    ra.split = h->counters + 5;
+    ra.split = h->counters + h->nlong_idx;
     ra.part = 1;
     const bool im = instance_major(h);
+    va::FwdArgs fb = fa;
+    va::RevArgs rb = ra;
+    // the bulk's parameter copy first, alone on the device (beside the long
+    // kernel's start it ran about half as fast); the long kernel has time to spare
+    if (im) {
+        if ((st = bulk_instance_major(h, p, &rb, 1, s))) return st;
+        fb.p = h->pim;
+    }
     if ((st = fork_side(h, s))) return st;
     // one block of three waves per long instance
     VA_HIP(h->ks.long_fused(fa, ra, VA_LONG_MAX, 192, s));
-    va::FwdArgs fb = fa;
-    va::RevArgs rb = ra;
-    if (im) {
-        if ((st = bulk_instance_major(h, p, &rb, 1, h->side))) return st;
-        fb.p = h->pim;
-    }
     VA_HIP(h->ks.fwd(fb, bulk_grid(h, h->lanes_fwd), h->block, h->side));
+    // The next solve's order and the caller's counts need every forward's counts:
+    // on the caller's stream after the long-instance kernel and the bulk's
+    // forward, beside the bulk's reverse (which still reads this solve's order),
+    // the order into the spare buffer.  A solve recorded into a graph builds it
+    // after the join into the same buffer, so that every replay reads the order
+    // the previous replay left.
+    const bool ahead = !h->capturing;
+    if (ahead) {
+        VA_HIP(hipEventRecord(h->ev_fwd, h->side));
+        VA_HIP(hipStreamWaitEvent(s, h->ev_fwd, 0));
+        if ((st = order_by_length(h, 2, s))) return st;
+        if ((st = copy_out(h, n_accepted, n_rejected, lane_status, s))) return st;
+    }
     VA_HIP(h->ks.rev(rb, bulk_grid(h, h->lanes_rev), h->block, h->side));
     if (im && (st = pbar_to_mu(h, dJdp, 1, cost == 0, h->side, h->pbar_set))) return st;
     if ((st = join_side(h, s))) return st;
-    return order_by_length(h, 1, s);
+    if (!ahead) {
+        if ((st = order_by_length(h, 1, s))) return st;
+        return copy_out(h, n_accepted, n_rejected, lane_status, s);
+    }
+    std::swap(h->order_fwd, h->order_fwd_next);
+    h->nlong_idx = 11 - h->nlong_idx;
+    return VA_OK;
 }
 
 // ---------------- plain form ----------------
@@ -1092,6 +1131,7 @@ va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_
     alloc((void **)&h->counters, sizeof(int32_t) * 8);
     alloc((void **)&h->order, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_fwd, sizeof(int32_t) * (size_t)pb->batch);
+    alloc((void **)&h->order_fwd_next, sizeof(int32_t) * (size_t)pb->batch);
     alloc((void **)&h->order_tmp, sizeof(int32_t) * 2 * VA_ORDER_BUCKETS);
     if (pb->n_obs > 0 || h->obs_ckpt) {
         alloc((void **)&h->obs_row, sizeof(int32_t) * std::max<size_t>(1, (size_t)B * pb->n_obs));
@@ -1147,6 +1187,7 @@ va_status create_handle(const va_problem *pb_in, const va::KernelSet *given, va_
         e = hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fwd, hipEventDisableTiming);
     }
     // no previous forward: zero attempt counts and no long instances
     if (e == hipSuccess) e = hipMemset(h->counters, 0, sizeof(int32_t) * 8);
@@ -1301,8 +1342,10 @@ va_status va_solve(va_handle *h, const double *y0, const double *p, int32_t cost
             h->graph_exec = nullptr;
         }
         VA_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        h->capturing = true;
         st = solve_enqueue(h, y0, p, cost, cost_weights, J, yT, dJdy0, dJdp, n_accepted,
                            n_rejected, lane_status, h->cap_stream);
+        h->capturing = false;
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(h->cap_stream, &graph);
         if (st) {
@@ -1351,8 +1394,11 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
         if ((st = read_max_steps(h, &smax, s))) return st;
         if ((st = reverse_plain_seed(h, p, dJdy0, dJdp, smax, s))) return st;
     } else if (split_solve(h)) {
-        if ((st = solve_split(h, yT, p, cost, cost_weights, J, dJdy0, dJdp, s))) return st;
-        h->have_forward = true;
+        // copies the counts to the caller's arrays itself
+        st = solve_split(h, yT, p, cost, cost_weights, J, dJdy0, dJdp, n_accepted, n_rejected,
+                         lane_status, s);
+        if (!st) h->have_forward = true;
+        return st;
     } else {
         if ((st = forward_persistent(h, yT, p, s))) return st;
         h->have_forward = true;
