@@ -114,10 +114,10 @@ __global__ void k_gather_rows(const double *ptraj, int64_t b, int64_t B, int fie
 constexpr int VA_TILE_B = 32;   // instances per block
 constexpr int VA_TILE_K = 128;  // parameters per pass: 32 x 129 doubles of LDS
 
-__global__ void __launch_bounds__(256) k_to_instance_major(const double *src, double *dst,
-                                                           int64_t P, int64_t B)
+__device__ __forceinline__ void to_instance_major_tile(const double *src, double *dst, int64_t P,
+                                                       int64_t B,
+                                                       double (&tile)[VA_TILE_B][VA_TILE_K + 1])
 {
-    __shared__ double tile[VA_TILE_B][VA_TILE_K + 1];
     const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B;
     const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
     const int bb = threadIdx.x & 31, kr = threadIdx.x >> 5;
@@ -131,6 +131,31 @@ __global__ void __launch_bounds__(256) k_to_instance_major(const double *src, do
             for (int k = lane; k < kn; k += 64) dst[(b0 + c) * P + k0 + k] = tile[c][k];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(256) k_to_instance_major(const double *src, double *dst,
+                                                           int64_t P, int64_t B)
+{
+    __shared__ double tile[VA_TILE_B][VA_TILE_K + 1];
+    to_instance_major_tile(src, dst, P, B, tile);
+}
+
+// A split solve's prologue in one launch: the parameter copy above, y(T) := y0
+// (when they differ) and cleared marks for the block's instances, and, in block
+// 0, the cleared work counters [0, 4).
+__global__ void __launch_bounds__(256) k_split_prologue(const double *p, double *pim, int64_t P,
+                                                        int64_t B, const double *y0, double *yT,
+                                                        int n, int32_t *marks, int32_t *counters)
+{
+    __shared__ double tile[VA_TILE_B][VA_TILE_K + 1];
+    to_instance_major_tile(p, pim, P, B, tile);
+    const int64_t b0 = (int64_t)blockIdx.x * VA_TILE_B;
+    const int nb = (int)((B - b0 < VA_TILE_B) ? B - b0 : VA_TILE_B);
+    const int bb = threadIdx.x & 31;
+    if (y0 != yT && bb < nb)
+        for (int i = threadIdx.x >> 5; i < n; i += 8) yT[(int64_t)i * B + b0 + bb] = y0[(int64_t)i * B + b0 + bb];
+    if ((int)threadIdx.x < nb) marks[b0 + threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0;
 }
 
 // dJ/dp sums of the row kernels, instance-major [nout][B][P], into mu
@@ -693,12 +718,22 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
 va_status copy_out(va_handle *h, int32_t *n_acc, int32_t *n_rej, int32_t *lane_status,
                    hipStream_t s);
 
-va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const double *cost_w,
-                      double *J, double *dJdy0, double *dJdp, int32_t *n_accepted,
-                      int32_t *n_rejected, int32_t *lane_status, hipStream_t s)
+va_status solve_split(va_handle *h, const double *y0, double *yT, const double *p, int cost,
+                      const double *cost_w, double *J, double *dJdy0, double *dJdp,
+                      int32_t *n_accepted, int32_t *n_rejected, int32_t *lane_status,
+                      hipStream_t s)
 {
-    va_status st = fill_i32(h->counters, 0, 4, s);
-    if (st) return st;
+    va_status st = VA_OK;
+    const int64_t B = h->pb.batch;
+    const bool im = instance_major(h);
+    if (!im) {
+        if ((st = fill_i32(h->counters, 0, 4, s))) return st;
+        if (yT != y0) {
+            hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)h->ks.n * B, 256)), dim3(256),
+                               0, s, yT, y0, (int64_t)h->ks.n * B);
+            VA_HIP(hipGetLastError());
+        }
+    }
     va::FwdArgs fa;
     fa.sp = solve_params(h);
     fa.y = yT;
@@ -729,14 +764,20 @@ va_status solve_split(va_handle *h, double *yT, const double *p, int cost, const
     ra.order = h->order_fwd;
     ra.split = h->counters + h->nlong_idx;
     ra.part = 1;
-    const bool im = instance_major(h);
     va::FwdArgs fb = fa;
     va::RevArgs rb = ra;
     // the bulk's parameter copy first, alone on the device (beside the long
-    // kernel's start it ran about half as fast); the long kernel has time to spare
+    // kernel's start it ran about half as fast; the long kernel has time to
+    // spare), in one launch with y(T) := y0 and the cleared marks and counters
     if (im) {
-        if ((st = bulk_instance_major(h, p, &rb, 1, s))) return st;
+        if ((st = pbar_instance_major(h, rb, 1))) return st;
+        rb.p = h->pim;
+        rb.pbar_set = h->pbar_set;
         fb.p = h->pim;
+        hipLaunchKernelGGL(k_split_prologue, dim3(blocks_for(B, VA_TILE_B)), dim3(256), 0, s, p,
+                           h->pim, (int64_t)h->ks.np, B, y0, yT, h->ks.n, h->pbar_set,
+                           h->counters);
+        VA_HIP(hipGetLastError());
     }
     if ((st = fork_side(h, s))) return st;
     // one block of three waves per long instance
@@ -1379,6 +1420,14 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
     hipStream_t s = (hipStream_t)stream;
     const int64_t B = h->pb.batch;
     const int n = h->ks.n, np = h->ks.np;
+    const bool split = h->pb.mode != VA_MODE_PLAIN && split_solve(h);
+    if (split) {
+        // copies y0 and the counts itself
+        st = solve_split(h, y0, yT, p, cost, cost_weights, J, dJdy0, dJdp, n_accepted,
+                         n_rejected, lane_status, s);
+        if (!st) h->have_forward = true;
+        return st;
+    }
     if (yT != y0) {
         hipLaunchKernelGGL(k_copy_f64, dim3(blocks_for((int64_t)n * B, 256)), dim3(256), 0, s, yT,
                            y0, (int64_t)n * B);
@@ -1393,12 +1442,6 @@ static va_status solve_enqueue(va_handle *h, const double *y0, const double *p, 
         int32_t smax = 0;
         if ((st = read_max_steps(h, &smax, s))) return st;
         if ((st = reverse_plain_seed(h, p, dJdy0, dJdp, smax, s))) return st;
-    } else if (split_solve(h)) {
-        // copies the counts to the caller's arrays itself
-        st = solve_split(h, yT, p, cost, cost_weights, J, dJdy0, dJdp, n_accepted, n_rejected,
-                         lane_status, s);
-        if (!st) h->have_forward = true;
-        return st;
     } else {
         if ((st = forward_persistent(h, yT, p, s))) return st;
         h->have_forward = true;
