@@ -112,3 +112,50 @@ def test_separate_calls_split_schedule():
     assert np.array_equal(res[0]["status"], o["status"])
     assert np.array_equal(res[0]["n_acc"][ok], o["n_acc"][ok])
     assert np.array_equal(res[0]["yT"][:, ok], o["yT"][:, ok])
+
+
+def test_interleaved_solves_and_separate_calls():
+    """A split solve builds the next solve's order beside its own reverse into a
+    second buffer and swaps the two when it returns, and copies the counts to the
+    caller there; separate va_forward / va_adjoint calls build their order after
+    the forward into the current buffer.  Any interleaving of the two on one handle
+    gives the outputs of the handle's first solve, bit for bit, and the oracle's."""
+    torch = _torch()
+    import vectorizedadjoint_amd as va
+    from vectorizedadjoint_amd import inputs as I
+    w = I.WORKLOADS["glv10"]
+    B, lanes = 3000, 120
+    inp = I.make_inputs(w, B)
+    o = run_oracle(w, inp, max_steps=1024)
+    assert int(((o["n_acc"] + o["n_rej"]) >= 256).sum()) > 0
+    y0, p = _dev(inp["y0"], torch), _dev(inp["p"], torch)
+    N, P = y0.shape[0], p.shape[0]
+    pb = va.Problem.from_workload(w, batch=B, max_steps=1024, lanes=lanes)
+    keys = ("yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status")
+    print(f"[gpu] interleaved solves and calls glv10 B={B} lanes={lanes}", flush=True)
+    with va.Solver(pb) as s:
+        def solve():
+            out = s.solve(y0, p, w.cost)
+            torch.cuda.synchronize()
+            return {k: v.cpu().numpy() for k, v in out.items()}
+
+        def calls():
+            y = y0.clone()
+            n_acc = torch.empty(B, dtype=torch.int32, device="cuda")
+            n_rej = torch.empty_like(n_acc)
+            st = torch.empty_like(n_acc)
+            s.forward(y, p, n_accepted=n_acc, n_rejected=n_rej, lane_status=st)
+            lam = torch.ones((1, N, B), dtype=torch.float64, device="cuda")
+            mu = torch.zeros((1, P, B), dtype=torch.float64, device="cuda")
+            s.adjoint(p, lam, mu)
+            torch.cuda.synchronize()
+            return {"yT": y.cpu().numpy(), "dJdy0": lam[0].cpu().numpy(),
+                    "dJdp": mu[0].cpu().numpy(), "n_acc": n_acc.cpu().numpy(),
+                    "n_rej": n_rej.cpu().numpy(), "status": st.cpu().numpy()}
+
+        ref = solve()
+        for step in (solve, solve, calls, solve, calls, calls, solve, solve):
+            got = step()
+            for k in (keys + ("J",) if step is solve else keys):
+                assert np.array_equal(got[k], ref[k], equal_nan=True), (step.__name__, k)
+    _compare(ref, o)
