@@ -28,15 +28,38 @@
 
 namespace va {
 
+// The time the controlled loop heads for after observation kobs - 1: observation
+// kobs's time, or t1 once every observation is taken.  It waits for its load at
+// once (s_waitcnt vmcnt(0)): it runs only where an observation is passed, and
+// with the load retired there the steps in between test the target without
+// waiting for the wave's earlier row stores (gfx9 counts loads and stores in
+// one in-order counter).
+__device__ __forceinline__ double obs_time(const double *obs_t, int K, int kobs, double t1)
+{
+    const double v = kobs < K ? obs_t[kobs] : t1;
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt 0; expcnt and lgkmcnt left at their maxima
+    return v;
+}
+
+// The stored-point index of observation kobs, or -1 (no point) once kobs < 0.
+__device__ __forceinline__ int32_t obs_point(const int32_t *orow, int kobs)
+{
+    return kobs >= 0 ? orow[kobs] : -1;
+}
+
 // Observation jumps at stored point `point` (state yn), newest observation first.
+// `next` holds obs_point(orow, kobs) and is reloaded only when kobs moves, so
+// the steps between observations test a register instead of waiting on a load.
 template <int N>
 __device__ __forceinline__ int obs_jumps(const int32_t *orow, const double *obs_y, int kobs,
-                                         int32_t point, const double *yn, double *lam)
+                                         int32_t &next, int32_t point, const double *yn,
+                                         double *lam)
 {
-    while (kobs >= 0 && orow[kobs] == point) {
+    while (kobs >= 0 && next == point) {
 #pragma unroll
         for (int i = 0; i < N; ++i) lam[i] = fma(2.0, yn[i] - obs_y[kobs * N + i], lam[i]);
         --kobs;
+        next = obs_point(orow, kobs);
     }
     return kobs;
 }
@@ -57,7 +80,9 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
 
     typename Sys::Params q;
     double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
-    double t = 0.0, dt = 0.0, dt_keep = 0.0, Jacc = 0.0;
+    // tnext: the next target, obs_time(a.obs_t, K, kobs, sp.t1), reloaded only
+    // when kobs moves, so a step's clipping test waits on no load
+    double t = 0.0, dt = 0.0, dt_keep = 0.0, Jacc = 0.0, tnext = 0.0;
     int64_t b = 0;
     int32_t count = 0, rej = 0, tries = 0, st = LANE_OK, kobs = 0, stride = 1, nck = 0;
     bool new_step = true, have_k0 = false, need = true, clipped = false;
@@ -90,6 +115,7 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
                 }
             }
             count = rej = tries = kobs = nck = 0;
+            tnext = obs_time(a.obs_t, K, 0, sp.t1);
             st = LANE_OK;
             new_step = true;
             have_k0 = false;
@@ -107,7 +133,7 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
             bool is_obs = false;
             for (;;) {
                 is_obs = kobs < K;
-                target = is_obs ? a.obs_t[kobs] : sp.t1;
+                target = tnext;
                 if (!is_obs || less_with_sign(t, target, dt)) break;
                 orow[kobs] = count;
 #pragma unroll
@@ -116,6 +142,7 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
                     Jacc = fma(d, d, Jacc);
                 }
                 ++kobs;
+                tnext = obs_time(a.obs_t, K, kobs, sp.t1);
             }
             if (!less_with_sign(t, target, dt)) {
                 finish = true;
@@ -124,7 +151,10 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
                 finish = true;
             } else {
                 if constexpr (Ckpt) {
-                    if (count % stride == 0) {
+                    // stride is a power of two (it starts at 1 and is only doubled
+                    // or halved), so this is count % stride == 0 without the
+                    // integer division's two dozen instructions on every step
+                    if ((count & (stride - 1)) == 0) {
                         if (nck >= a.st.slots) {
                             // drop every other checkpoint, double the spacing
                             for (int m = 1; m < a.st.slots / 2; ++m)
@@ -240,6 +270,7 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
     double y[N], Kst[Tab::SF][N], ynew[N], xerr[N];
     double t = c[0], dt = c[1], dt_keep = 0.0;
     int kobs = (int)c[2];
+    double tnext = obs_time(obs_t, K, kobs, sp.t1);  // as in the forward
 #pragma unroll
     for (int i = 0; i < N; ++i) y[i] = c[3 + i];
     // f(y) here equals the FSAL value the forward carried into this step: the
@@ -254,9 +285,10 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
             bool is_obs;
             for (;;) {
                 is_obs = kobs < K;
-                target = is_obs ? obs_t[kobs] : sp.t1;
+                target = tnext;
                 if (!is_obs || less_with_sign(t, target, dt)) break;
                 ++kobs;
+                tnext = obs_time(obs_t, K, kobs, sp.t1);
             }
             if constexpr (Rows) {
                 r = seg + (size_t)k * FS * ss;
@@ -351,13 +383,14 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
         for (int k = 0; k < P; ++k) pbar[k] = 0.0;
         const int32_t cnt = a.n_acc[b];
         const int32_t *orow = a.st.obs_row + (size_t)b * (size_t)K;
-        int kobs = obs_jumps<N>(orow, a.obs_y, K - 1, cnt, yT, lam);
+        int32_t onext = obs_point(orow, K - 1);
+        int kobs = obs_jumps<N>(orow, a.obs_y, K - 1, onext, cnt, yT, lam);
         if constexpr (!Ckpt) {
             const double *rows = a.st.traj + (size_t)b * (size_t)(sp.cap + 1) * F;
             for (int32_t n = cnt - 1; n >= 0; --n) {
                 const double *r = rows + (size_t)n * F;
                 back_prop_step<Sys, Tab>(q, r[1], r + 2, lam, pbar);
-                kobs = obs_jumps<N>(orow, a.obs_y, kobs, n, r + 2, lam);
+                kobs = obs_jumps<N>(orow, a.obs_y, kobs, onext, n, r + 2, lam);
             }
         } else {
             const int32_t stride = a.st.stride[b];
@@ -366,13 +399,24 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
             // reverse `len` rows from checkpoint c; the first is point n0
             auto reverse_rows = [&](const double *c, int32_t n0, int32_t len) {
                 recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, c, len, seg, ss);
-                for (int32_t k = len - 1; k >= 0; --k) {
+                // row k-1 is loaded while step k is reversed, so no step waits
+                // on its own row's loads
+                double hn = 0.0, yn[N] = {};
+                auto load_row = [&](int32_t k) {
                     const double *r = seg + (size_t)k * FS * ss;
+                    hn = r[0];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) yn[i] = r[(size_t)(1 + i) * ss];
+                };
+                if (len > 0) load_row(len - 1);
+                for (int32_t k = len - 1; k >= 0; --k) {
+                    const double hk = hn;
                     double yk[N];
 #pragma unroll
-                    for (int i = 0; i < N; ++i) yk[i] = r[(size_t)(1 + i) * ss];
-                    back_prop_step<Sys, Tab>(q, r[0], yk, lam, pbar);
-                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, n0 + k, yk, lam);
+                    for (int i = 0; i < N; ++i) yk[i] = yn[i];
+                    if (k > 0) load_row(k - 1);
+                    back_prop_step<Sys, Tab>(q, hk, yk, lam, pbar);
+                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, onext, n0 + k, yk, lam);
                 }
             };
             for (int32_t j = nseg - 1; j >= 0; --j) {
