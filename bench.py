@@ -426,7 +426,7 @@ def main(argv=None) -> int:
                             "first solve runs in index order: schedule.cold_index_order)"},
         "roofline": {"bound": "fp64-valu", "compute": "fp64 vector ALU (its peak equals the "
                      "FP64 matrix peak on MI355X)", "kernel": "solve",
-                     "kernels": "k_glv_rows_long (one block of two waves per long instance) "
+                     "kernels": "k_glv_rows_long (one block of three waves per long instance) "
                                 "beside k_glv_rows2_fwd then k_glv_rows_rev",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
