@@ -75,6 +75,37 @@ def test_checkpointed_reverse_is_bit_identical_to_full_store():
             assert np.array_equal(full[k], ck[k]), (slots, k)
 
 
+def test_reverse_in_wave_pairs_with_failed_instances_and_few_lanes():
+    """The checkpointed reverse in pairs of waves (one replaying segments while
+    the other reverses the previous one; blocks of 256 threads) against the
+    one-wave reverse (blocks of 64 threads) and the full store: 128 lanes for
+    512 instances, so every lane pair takes several instances one after the
+    other, and a step cap that ends the longest instances with MAX_STEPS
+    (NaN gradients, no segments replayed for them).  Every output identical."""
+    B = 512
+    torch, O, va, w, inp, t_obs, y_obs, pb = _setup(B, t1=12.0, K=30)
+
+    def run(**kw):
+        prob = va.Problem.from_workload(w, batch=B, t1=12.0, n_obs=t_obs.size, max_steps=700,
+                                        lanes=128, **kw)
+        print(f"[gpu] observed wave pairs B={B} {kw}", flush=True)
+        with va.Solver(prob) as s:
+            dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+            out = s.solve_observed(dev(inp["y0"]), dev(inp["p"]), dev(t_obs), dev(y_obs))
+            torch.cuda.synchronize()
+            return {k: v.cpu().numpy() for k, v in out.items()}
+
+    full = run(block_size=256)
+    pairs = run(block_size=256, checkpoint_slots=8)
+    single = run(block_size=64, checkpoint_slots=8)
+    failed = full["status"] != 0
+    assert failed.any() and (~failed).any()
+    assert np.isnan(full["dJdp"][:, failed]).all()
+    for k in ("J", "yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status"):
+        assert np.array_equal(full[k], pairs[k], equal_nan=True), k
+        assert np.array_equal(full[k], single[k], equal_nan=True), k
+
+
 def test_two_level_checkpoints_are_bit_identical_to_full_store():
     """Two levels of checkpoints (VA_FLAG_CHECKPOINT_2LEVEL): the reverse replays
     each long segment once to keep level-2 checkpoints, then recomputes its rows

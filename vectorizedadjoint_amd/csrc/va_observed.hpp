@@ -344,10 +344,253 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
     }
 }
 
+// ---------------------------------------------------------------------------
+// Checkpointed reverse in pairs of waves (one level of checkpoints)
+// ---------------------------------------------------------------------------
+// A lane's reverse is two dependent chains one after the other, segment by
+// segment: the replay of the controlled loop from the segment's checkpoint and
+// the backward sweep over the replayed rows.  Here wave 2p of a block replays
+// while wave 2p+1 sweeps backwards, for the same 64 instances (one per lane):
+// in iteration i each replay lane rebuilds one segment into the row buffer of
+// parity i & 1 and records which (instance, segment) it was; the backward lane
+// of the same index reverses iteration i's segment while the replay lane
+// rebuilds iteration i+1's into the other buffer.  The two buffers are the row
+// buffers the one-wave reverse gives the two waves' threads, so the memory is
+// the same.  The waves meet through two counters in LDS (iterations replayed,
+// iterations reversed), written and polled by whole waves, so no lane waits
+// on another lane of its own wave; every wait gives up after VA_OBS_SPIN_LIMIT
+// polls, and then each wave writes NaN outputs for the instances it holds and
+// stops.  The replay runs at most two iterations ahead.  Results are those of
+// the one-wave reverse: the same rows, reversed in the same order.
+constexpr int VA_OBS_SPIN_LIMIT = 1 << 22;  // polls (about 0.1 s) before a wait gives up
+
+struct ObsPipe {
+    int item_b[2][64];  // instance of each lane in the iterations of each parity (-1: idle)
+    int item_j[2][64];  // its segment (-1: none; the instance's outputs only)
+    int rep;            // iterations replayed
+    int con;            // iterations reversed: a buffer is free again
+    int abort_;         // a wait gave up
+};
+
+__device__ __forceinline__ int pipe_load(int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void pipe_store(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Wait until *ctr > k (the same word for every lane, so the whole wave leaves
+// the loop together).  Returns false when the other wave gave up or, after
+// VA_OBS_SPIN_LIMIT polls, gives up itself.
+__device__ __forceinline__ bool pipe_wait(ObsPipe &P, int *ctr, int k)
+{
+    for (int spins = 0; spins < VA_OBS_SPIN_LIMIT; ++spins) {
+        if (pipe_load(ctr) > k) {
+            // the other wave's buffer and item writes before its count come first
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            return true;
+        }
+        if (pipe_load(&P.abort_)) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    pipe_store(&P.abort_, 1);
+    return false;
+}
+
+template <int N, int P>
+__device__ __forceinline__ void obs_nan_outputs(const ObsRevArgs &a, int64_t b)
+{
+    const int64_t B = a.sp.B;
+    const double qnan = __builtin_nan("");
+#pragma unroll
+    for (int i = 0; i < N; ++i) a.dJdy0[(int64_t)i * B + b] = qnan;
+    for (int k = 0; k < P; ++k) a.dJdp[(int64_t)k * B + b] = qnan;
+}
+
+// Replay wave: takes the instances (longest first) and rebuilds their segments,
+// last first.
+template <class Sys, class Tab>
+__device__ void obs_rev_replay(const ObsRevArgs &a, ObsPipe &Pp, int pair, int lane)
+{
+    constexpr int N = Sys::N, P = Sys::P, FC = N + 3;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int K = a.K;
+    const int64_t ss = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tbase = (int64_t)blockIdx.x * blockDim.x + pair * 128 + lane;
+    typename Sys::Params q;
+    int64_t b = -1;
+    int32_t cnt = 0, stride = 1, j = -1;
+    bool done = false, have = false;
+    for (int it = 0;; ++it) {
+        // this lane's item: the next segment of its instance, or a new instance
+        bool seg_item = false;
+        int64_t ib = -1;
+        if (!done && !have) {
+            const int64_t pos = atomicAdd(a.counter, 1);
+            if (pos >= B) {
+                done = true;
+            } else {
+                b = a.order ? a.order[pos] : pos;
+                if (a.status[b] == LANE_OK) {
+                    Sys::load(q, a.p, B, b, sp.kconst);
+                    cnt = a.n_acc[b];
+                    stride = a.st.stride[b];
+                    j = (cnt + stride - 1) / stride - 1;
+                    have = j >= 0;
+                }
+                ib = b;  // an instance without segments: its outputs only
+            }
+        }
+        if (have) {
+            ib = b;
+            seg_item = true;
+        }
+        if (__builtin_amdgcn_ballot_w64(!done) == 0) {
+            // the final iteration, all lanes idle: the backward wave stops at it
+            if (!pipe_wait(Pp, &Pp.con, it - 2)) return;
+            Pp.item_b[it & 1][lane] = -1;
+            Pp.item_j[it & 1][lane] = -1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            pipe_store(&Pp.rep, it + 1);
+            return;
+        }
+        // the buffer of this parity was read by the backward wave's iteration it - 2
+        if (!pipe_wait(Pp, &Pp.con, it - 2)) {
+            // NaN outputs for every instance this lane handed over or holds
+            for (int d = 1; d <= 2; ++d)
+                if (it - d >= 0 && Pp.item_b[(it - d) & 1][lane] >= 0)
+                    obs_nan_outputs<N, P>(a, Pp.item_b[(it - d) & 1][lane]);
+            if (ib >= 0) obs_nan_outputs<N, P>(a, ib);
+            return;
+        }
+        Pp.item_b[it & 1][lane] = (int)ib;
+        Pp.item_j[it & 1][lane] = seg_item ? j : -1;
+        if (seg_item) {
+            const double *c = a.st.ck + ((size_t)b * (size_t)a.st.slots + (size_t)j) * FC;
+            const int32_t n0 = j * stride;
+            const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
+            recompute_segment<Sys, Tab>(q, sp, a.obs_t, K, c, len,
+                                        a.st.seg + tbase + (it & 1) * 64, ss);
+            if (--j < 0) have = false;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        pipe_store(&Pp.rep, it + 1);
+    }
+}
+
+// Backward wave: reverses each replayed segment and writes the outputs.
+template <class Sys, class Tab>
+__device__ void obs_rev_backward(const ObsRevArgs &a, ObsPipe &Pp, int pair, int lane)
+{
+    constexpr int N = Sys::N, P = Sys::P, FS = N + 1;
+    const SolveParams sp = a.sp;
+    const int64_t B = sp.B;
+    const int K = a.K;
+    const int64_t ss = (int64_t)gridDim.x * blockDim.x;
+    const int64_t tbase = (int64_t)blockIdx.x * blockDim.x + pair * 128 + lane;
+    typename Sys::Params q;
+    double lam[N], pbar[P];
+    int64_t cur = -1;  // the instance in progress
+    int32_t stride = 1, cnt = 0, onext = -1;
+    int kobs = -1;
+    const int32_t *orow = nullptr;
+    // state for instance b, reversed from its end
+    auto start = [&](int64_t b) {
+        Sys::load(q, a.p, B, b, sp.kconst);
+        double yT[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            lam[i] = 0.0;
+            yT[i] = a.yT[(int64_t)i * B + b];
+        }
+        if (a.cost) a.J[b] = terminal_cost<N>(a.cost, a.cost_w, yT, lam);
+#pragma unroll
+        for (int k = 0; k < P; ++k) pbar[k] = 0.0;
+        cnt = a.n_acc[b];
+        stride = a.st.stride[b];
+        orow = a.st.obs_row + (size_t)b * (size_t)K;
+        onext = obs_point(orow, K - 1);
+        kobs = obs_jumps<N>(orow, a.obs_y, K - 1, onext, cnt, yT, lam);
+        cur = b;
+    };
+    auto finish = [&](int64_t b) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) a.dJdy0[(int64_t)i * B + b] = lam[i];
+        for (int k = 0; k < P; ++k) a.dJdp[(int64_t)k * B + b] = 0.0 + pbar[k];
+        cur = -1;
+    };
+    for (int it = 0;; ++it) {
+        if (!pipe_wait(Pp, &Pp.rep, it)) {
+            if (cur >= 0) obs_nan_outputs<N, P>(a, cur);
+            return;
+        }
+        const int ib = Pp.item_b[it & 1][lane];
+        const int ij = Pp.item_j[it & 1][lane];
+        if (__builtin_amdgcn_ballot_w64(ib >= 0) == 0) return;  // the replay wave is done
+        if (ib >= 0) {
+            if (ij < 0) {
+                // no segments: a failed instance, or one with no accepted step
+                if (a.status[ib] != LANE_OK) {
+                    obs_nan_outputs<N, P>(a, ib);
+                } else {
+                    start(ib);
+                    finish(ib);
+                }
+            } else {
+                if (ib != cur) start(ib);
+                const int32_t n0 = ij * stride;
+                const int32_t len = (cnt - n0 < stride) ? cnt - n0 : stride;
+                const double *seg = a.st.seg + tbase + (it & 1) * 64;
+                // row k-1 is loaded while step k is reversed
+                double hn = 0.0, yn[N] = {};
+                auto load_row = [&](int32_t k) {
+                    const double *r = seg + (size_t)k * FS * ss;
+                    hn = r[0];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) yn[i] = r[(size_t)(1 + i) * ss];
+                };
+                if (len > 0) load_row(len - 1);
+                for (int32_t k = len - 1; k >= 0; --k) {
+                    const double hk = hn;
+                    double yk[N];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) yk[i] = yn[i];
+                    if (k > 0) load_row(k - 1);
+                    back_prop_step<Sys, Tab>(q, hk, yk, lam, pbar);
+                    kobs = obs_jumps<N>(orow, a.obs_y, kobs, onext, n0 + k, yk, lam);
+                }
+                if (ij == 0) finish(ib);
+            }
+        }
+        // this iteration's buffer may be rebuilt
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        pipe_store(&Pp.con, it + 1);
+    }
+}
+
 template <class Sys, class Tab, bool Ckpt>
 __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
 {
-    if constexpr (Ckpt) load_ctrl_tables();
+    if constexpr (Ckpt) {
+        load_ctrl_tables();
+        if (a.st.levels == 1 && (blockDim.x & 127) == 0) {
+            // pairs of waves (the condition is the same for the whole grid)
+            __shared__ ObsPipe pipes[2];
+            const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            if (lane == 0 && (w & 1) == 0) {
+                pipes[w >> 1].rep = 0;
+                pipes[w >> 1].con = 0;
+                pipes[w >> 1].abort_ = 0;
+            }
+            __syncthreads();
+            if (w & 1) obs_rev_backward<Sys, Tab>(a, pipes[w >> 1], w >> 1, lane);
+            else obs_rev_replay<Sys, Tab>(a, pipes[w >> 1], w >> 1, lane);
+            return;
+        }
+    }
     constexpr int N = Sys::N, P = Sys::P;
     constexpr int F = N + 2, FC = N + 3, FS = N + 1;
     const SolveParams sp = a.sp;
