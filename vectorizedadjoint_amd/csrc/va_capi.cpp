@@ -697,7 +697,9 @@ va_status reverse_persistent(va_handle *h, const double *p, int cost, const doub
         a.split = h->counters + h->nlong_idx;
         a.part = 1;
         if ((st = fork_side(h, s))) return st;
-        VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
+        // one seed: a block of three waves per long instance, as in a solve
+        if (nseed == 1) VA_HIP(h->ks.long_rev(a, VA_LONG_MAX, 192, s));
+        else VA_HIP(h->ks.long_rev(a, VA_LONG_MAX / 4, 256, s));
         va::RevArgs ab = a;
         if (im && (st = bulk_instance_major(h, p, &ab, nseed, h->side))) return st;
         VA_HIP(h->ks.rev(ab, bulk_grid(h, h->lanes_rev), h->block, h->side));

@@ -1255,7 +1255,10 @@ __global__ void __launch_bounds__(192) k_glv_rows_long(FwdArgs fa, RevArgs ra)
 }
 
 // The same instances in a forward alone (va_forward) and in a reverse alone
-// (va_adjoint, positions [0, *split) of the order of the last forward).
+// (va_adjoint, positions [0, *split) of the order of the last forward).  The
+// forward runs one wave per instance, four to a block: with one long forward
+// per CU (the long kernel's LDS reservation) the bulk forward beside it lost
+// more than the long forwards gained (forward call 2.20-2.25 -> 2.38-2.39 ms).
 template <int NN, class Tab>
 __global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs fa)
 {
@@ -1263,6 +1266,30 @@ __global__ void __launch_bounds__(256) k_glv_rows_long_fwd(FwdArgs fa)
     const int64_t w = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= *fa.split || w >= fa.sp.B) return;
     glv_rows_fwd_body<NN, Tab, true, true>(fa, fa.order[w]);
+}
+
+// The reverse alone for one seed, as in k_glv_rows_long: one block of three
+// waves per instance (the adjoint recursion, the stage values ahead of it, the
+// dJ/dr and dJ/dA accumulation), one block per CU.
+template <int NN, class Tab>
+__global__ void __launch_bounds__(192) k_glv_rows_long_adj(RevArgs ra)
+{
+    __shared__ StageRing<NN, Tab> ring;
+    __shared__ double lds_pad[VA_LONG_LDS_PAD / 8];
+    if (ra.sp.B < 0) lds_pad[threadIdx.x] = 0.0;  // never: keeps the reservation
+    const int64_t w = blockIdx.x;
+    if (w >= *ra.split || w >= ra.sp.B) return;  // the whole block
+    const int64_t b = ra.order[w];
+    if (threadIdx.x == 0) {
+        ring.avail = 0;
+        ring.uavail = 0;
+        ring.done = 0;
+        ring.abort = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) glv_long_backward<NN, Tab>(ra, b, ring);
+    else if (threadIdx.x < 128) glv_long_stage<NN, Tab>(ra, b, ring);
+    else glv_long_accumulate<NN, Tab>(ra, b, ring);
 }
 
 template <int NN, class Tab>
@@ -1518,9 +1545,14 @@ struct LaunchGlvRows {
         hipLaunchKernelGGL((k_glv_rows_long_fwd<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
+    // block 192: one block of three waves per instance (one seed); otherwise one
+    // wave per instance (any number of seeds)
     static hipError_t lng_rev(const RevArgs &a, int grid, int block, hipStream_t s)
     {
-        hipLaunchKernelGGL((k_glv_rows_long_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        if (block == 192)
+            hipLaunchKernelGGL((k_glv_rows_long_adj<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_glv_rows_long_rev<NN, Tab>), dim3(grid), dim3(block), 0, s, a);
         return hipGetLastError();
     }
     static hipError_t pfwd(const PlainFwdArgs &a, int grid, int block, hipStream_t s)

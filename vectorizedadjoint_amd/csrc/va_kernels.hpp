@@ -178,7 +178,9 @@ struct KernelSet {
     // forward then reverse of each long-running instance, one block of three waves per
     // instance: grid = instances, block = 192 (null where absent)
     hipError_t (*long_fused)(const FwdArgs &, const RevArgs &, int grid, int block, hipStream_t);
-    // the same instances' forward alone and reverse alone (null where absent)
+    // the same instances' forward alone (one wave per instance) and reverse
+    // alone (block 192: three waves per instance, one seed; block 256: one wave
+    // per instance, any seeds) (null where absent)
     hipError_t (*long_fwd)(const FwdArgs &, int grid, int block, hipStream_t);
     hipError_t (*long_rev)(const RevArgs &, int grid, int block, hipStream_t);
     hipError_t (*plain_fwd)(const PlainFwdArgs &, int grid, int block, hipStream_t);
