@@ -87,7 +87,10 @@ def test_shared_full_batch_slice_matches_oracle():
     tol 1e-8: 1024 tiles over every CU, each workgroup taking several), with the
     per-instance outputs of a 512-instance slice compared to the oracle (the whole
     batch would take the oracle half a minute; the batch-summed dJ/dA is covered
-    against the whole-batch oracle by the smaller cases above)."""
+    against the whole-batch oracle by the smaller cases above).  The handle solves
+    twice, and every per-instance output of the second solve must equal the
+    first's bit for bit (the batch-summed dJ/dA to 1e-12: the reverse takes its
+    tiles longest-first, in an order that may differ within a bucket)."""
     import os
     import torch
     import oracle as O
@@ -103,10 +106,16 @@ def test_shared_full_batch_slice_matches_oracle():
     print(f"[gpu] shared GLV N=128 full batch B={B}", flush=True)
     dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     with va.Solver(prob) as s:
-        out = s.solve_shared(dev(inp["y0"]), dev(inp["p"]),
-                             dev(inp["p_shared"].reshape(128, 128)), K.COST_SUM)
-        torch.cuda.synchronize()
-        g = {k: v.cpu().numpy() for k, v in out.items()}
+        runs = []
+        for _ in range(2):
+            out = s.solve_shared(dev(inp["y0"]), dev(inp["p"]),
+                                 dev(inp["p_shared"].reshape(128, 128)), K.COST_SUM)
+            torch.cuda.synchronize()
+            runs.append({k: v.cpu().numpy() for k, v in out.items()})
+    g0, g = runs
+    for k in ("status", "n_acc", "n_rej", "yT", "J", "dJdy0", "dJdr"):
+        assert np.array_equal(g0[k], g[k]), k
+    assert np.max(np.abs(g["dJdA"] - g0["dJdA"])) <= 1e-12 * np.max(np.abs(g0["dJdA"]))
     assert np.all(np.isfinite(g["dJdA"]))
     pb = O.problem(w.system, w.n, w.np, w.tableau, t0=w.t0, t1=w.t1, dt0=w.dt0,
                    abs_tol=w.tol, rel_tol=w.tol, max_steps=512)
