@@ -106,6 +106,22 @@ def test_reverse_in_wave_pairs_with_failed_instances_and_few_lanes():
         assert np.array_equal(full[k], single[k], equal_nan=True), k
 
 
+@pytest.mark.parametrize("tab", ["cash_karp54", "fehlberg78"])
+def test_checkpointed_reverse_other_tableaus(tab):
+    """The checkpointed reverse (in wave pairs) with the other embedded tableaus:
+    Cash-Karp 5(4) (no FSAL: the replay evaluates f(y) at each step start) and
+    Fehlberg 7(8) (13 stages), against the full store, every output identical."""
+    from vectorizedadjoint_amd import constants as K
+    B = 256
+    torch, O, va, w, inp, t_obs, y_obs, pb = _setup(B, t1=8.0, K=20)
+    tid = {"cash_karp54": K.TAB_CASH_KARP54, "fehlberg78": K.TAB_FEHLBERG78}[tab]
+    full = _gpu(torch, va, w, inp, t_obs, y_obs, 8.0, tableau=tid)
+    ck = _gpu(torch, va, w, inp, t_obs, y_obs, 8.0, tableau=tid, checkpoint_slots=6)
+    assert (full["status"] == 0).all()
+    for k in ("J", "yT", "dJdy0", "dJdp", "n_acc", "n_rej", "status"):
+        assert np.array_equal(full[k], ck[k]), k
+
+
 def test_two_level_checkpoints_are_bit_identical_to_full_store():
     """Two levels of checkpoints (VA_FLAG_CHECKPOINT_2LEVEL): the reverse replays
     each long segment once to keep level-2 checkpoints, then recomputes its rows
