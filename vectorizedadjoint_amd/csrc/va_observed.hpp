@@ -103,15 +103,17 @@ __global__ void __launch_bounds__(256) k_obs_fwd(ObsFwdArgs a)
             stride = 1;
             if constexpr (Ckpt) {
                 // Start from the spacing the previous solve of this instance reached
-                // (its step count is still in n_acc until this one finishes), halved
-                // for safety, instead of doubling up to it: each doubling moves half
-                // the checkpoints, and with one lane per instance the whole wave
-                // waits for any lane that moves them.  Any spacing gives the same
+                // (its step count is still in n_acc until this one finishes): the
+                // smallest with room for that many steps, so that a solve taking
+                // the same steps again never fills the slots.  Each doubling moves
+                // half the checkpoints, one lane at a time, and with one lane per
+                // instance the whole wave waits for any lane that moves them (half
+                // that spacing, which filled the slots once per solve, cost 10% of
+                // the observed Van der Pol solve).  Any spacing gives the same
                 // results: the reverse replays the forward from each checkpoint.
                 if (a.order) {
                     const int32_t prev = a.n_acc[b];
                     while ((int64_t)stride * a.st.slots <= prev) stride *= 2;
-                    stride = stride > 1 ? stride / 2 : 1;
                 }
             }
             count = rej = tries = kobs = nck = 0;
