@@ -347,7 +347,8 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
 }
 
 // ---------------------------------------------------------------------------
-// Checkpointed reverse in pairs of waves (one level of checkpoints)
+// Checkpointed reverse in pairs of waves (one level of checkpoints, segments
+// of at most VA_OBS_PAIR_MAX_ROWS rows)
 // ---------------------------------------------------------------------------
 // A lane's reverse is two dependent chains one after the other, segment by
 // segment: the replay of the controlled loop from the segment's checkpoint and
@@ -364,7 +365,12 @@ __device__ void recompute_segment(const typename Sys::Params &q, const SolvePara
 // polls, and then each wave writes NaN outputs for the instances it holds and
 // stops.  The replay runs at most two iterations ahead.  Results are those of
 // the one-wave reverse: the same rows, reversed in the same order.
-constexpr int VA_OBS_SPIN_LIMIT = 1 << 22;  // polls (about 0.1 s) before a wait gives up
+// A wait lasts one iteration of the other wave: one segment replayed or
+// reversed, at most VA_OBS_PAIR_MAX_ROWS steps (longer segments take the
+// one-wave reverse), tens of milliseconds; a wait gives up only after some
+// seconds of polls.
+constexpr int VA_OBS_SPIN_LIMIT = 1 << 26;
+constexpr int VA_OBS_PAIR_MAX_ROWS = 8192;
 
 struct ObsPipe {
     int item_b[2][64];  // instance of each lane in the iterations of each parity (-1: idle)
@@ -578,7 +584,8 @@ __global__ void __launch_bounds__(256) k_obs_rev(ObsRevArgs a)
 {
     if constexpr (Ckpt) {
         load_ctrl_tables();
-        if (a.st.levels == 1 && (blockDim.x & 127) == 0) {
+        if (a.st.levels == 1 && a.st.seg_rows <= VA_OBS_PAIR_MAX_ROWS &&
+            (blockDim.x & 127) == 0) {
             // pairs of waves (the condition is the same for the whole grid)
             __shared__ ObsPipe pipes[2];
             const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
