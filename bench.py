@@ -149,6 +149,16 @@ def cpu_baseline(w, inputs, max_steps, target_s=10.0, max_reps=20):
                               "sample": f"first {k1} instances, {t1:.1f} s, 1 thread"}}
 
 
+def measured_traffic(name):
+    """HBM bytes per solve from the calibrated FETCH_SIZE / WRITE_SIZE runs kept
+    under profiles/ (their `solve` entry), or None where there are none."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("solve")
+
+
 def bench_observed(a, torch, va, I):
     """The observed Van der Pol workload: forward with stops at 50 observation times,
     running cost, adjoint with lambda jumps, checkpointed under a 4 GB trajectory budget."""
@@ -221,7 +231,8 @@ def bench_observed(a, torch, va, I):
                    "trajectory_budget_bytes": budget, "instances_ok": n_ok},
         "roofline": {"bound": "fp64-valu", "compute": "fp64 vector ALU", "kernel": "solve",
                      "achieved": fl / sec / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": fl / sec / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": fl / sec / 1e12 / FP64_PEAK_TFLOPS,
+                     "traffic": measured_traffic("traffic_vdp_obs.json"),
                      "flop_per_solve": fl, "note": "forward counted twice (recompute)"},
         "cpu_baseline": cpu,
         "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
@@ -402,11 +413,7 @@ def main(argv=None) -> int:
         log("[bench] CPU baseline")
         cpu = cpu_baseline(w, inp, MAX_STEPS)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_glv10.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("solve")
+    traffic = measured_traffic("traffic_glv10.json")
 
     res = {
         "metric": METRIC, "value": n_ok / sec, "unit": "gradients/s", "n_gpus": 1,
