@@ -305,7 +305,8 @@ def bench_shared(a, torch, va, I):
         "roofline": {"bound": "mfma", "compute": "fp64 MFMA 16x16x4", "kernel": "solve",
                      "achieved": fl_issued / sec / 1e12, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": fl_issued / sec / 1e12 / FP64_PEAK_TFLOPS,
-                     "traffic": None, "flop_per_solve": fl_issued},
+                     "traffic": measured_traffic("traffic_glv128_shared.json"),
+                     "flop_per_solve": fl_issued},
         "cpu_baseline": cpu,
         "steps_per_instance": {"accepted_mean": float(n_acc.mean()),
                                "rejected_mean": float(n_rej.mean())},
